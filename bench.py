@@ -1,0 +1,293 @@
+"""Benchmark of the hot path on one MI355X.
+
+Default workload (the kernel benchmark configuration): a 3D homogeneous P2 slope (10 m high, 45
+deg face, 40 m out of plane, ~117k 10-node tetrahedra, 11-point rule, ~1.29M points, ~0.5M dofs).
+Inputs resident in HBM: branch-stratified strains (70% elastic, 30% plastic over smooth face,
+left/right edge and apex) at every point, the geometry (dphi, w) and the fixed CSR pattern.
+
+One step = the Newton-iteration hot path on those inputs:
+  return mapping (E -> S, DS)  +  internal force (S -> F)  +  tangent assembly into the fixed
+  pattern with the regularised operator K_r = (1-r)K + r K_el and its FP32 copy.
+`value` = algorithmic HBM bytes of the step / step time (GB/s).
+
+Prints one JSON line.  Rank/world handling: one GPU only (--gpus > 1 exits with a message).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "SSR wall time to FoS (learned+FP32 vs plain GPU) at 1e-3; assembly HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def _log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def build_workload(h: float, nz: int, n_unique: int, seed: int):
+    import torch
+
+    from slope_stability_petsc4py_amd import mesh, workloads
+    from slope_stability_petsc4py_amd.constitutive import ConstitutiveOperator
+
+    t0 = time.time()
+    m = mesh.slope_3d(h=h, nz=nz, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0, z_len=40.0)
+    op = ConstitutiveOperator.from_mesh(m, [workloads.HOMO], Davis_type="B")
+    _log(f"mesh+pattern {time.time() - t0:.1f}s: {m.n_elem} tets, {op.n_int} points, {op.pattern.n_rows} dofs, "
+         f"nnz {op.pattern.host.nnz}")
+    E_u, br_u = workloads.stratified_strains(n_unique, seed=seed)
+    reps = -(-op.n_int // n_unique)
+    E = np.ascontiguousarray(np.tile(E_u, (reps, 1))[: op.n_int])
+    br = np.tile(br_u, reps)[: op.n_int]
+    op.reduction(1.0)
+    op._E.copy_(torch.from_numpy(E))
+    torch.cuda.synchronize()
+    return m, op, E, br
+
+
+def algorithmic_bytes(op) -> dict:
+    """Per-launch algorithmic HBM bytes of each kernel of the step."""
+    n = op.n_int
+    pat = op.pattern
+    nnz = pat.host.nnz
+    n_p = pat.n_p
+    return {
+        # E 48 + (c_bar, sin_phi, shear, lame) 32 in; S 48 + DS 288 out
+        "constitutive": n * (48 + 32 + 48 + 288),
+        # S 48 + dphi 3*n_p*8 + w 8 per point in; fe out; then fe in + F out
+        "force": n * (48 + 24 * n_p + 8) + pat.n_elem * pat.nld * 8 * 2 + pat.n_rows * 8,
+        # DS 288 + dphi 3*n_p*8 + w 8 per point in; K_el nnz*8 in; K_r nnz*8 + FP32 nnz*4 out
+        "tangent": n * (288 + 24 * n_p + 8) + nnz * (8 + 8 + 4),
+        # dphi + u gather (3*n_p*8 per element, counted once) + E out
+        "strain": n * (24 * n_p + 48) + pat.n_elem * pat.nld * 8,
+        # CSR FP64: values 8 + index 4 per nnz, x 8 (once) + y 8 + indptr 4 per row
+        "spmv": nnz * 12 + pat.n_rows * 20,
+    }
+
+
+def run_gpu(args):
+    import torch
+
+    from slope_stability_petsc4py_amd import _lib
+    from slope_stability_petsc4py_amd.device import ptr, stream_ptr
+
+    m, op, E_host, br = build_workload(args.h, args.nz, args.unique, args.seed)
+    pat = op.pattern
+    dev = op._E.device
+    S, DS = op._S, op._DS
+    K = op.pattern.new_csr(with32=True)
+    fe, F = op._fe, op._F
+    st = stream_ptr()
+    r = 1e-4
+
+    def k_const():
+        _lib.call("ssx_constitutive_3d", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear), ptr(op.lame),
+                  op.n_int, ptr(S), ptr(DS), None, None, st)
+
+    def k_force():
+        pat.force(S, fe, F)
+
+    def k_tangent():
+        pat.tangent(DS, K.values, K.values32, r=r, elastic=op.elastic_values)
+
+    u = torch.from_numpy(np.random.default_rng(1).normal(0, 1e-3, pat.n_rows)).to(dev)
+    E_tmp = torch.empty_like(op._E)
+    y = torch.empty(pat.n_rows, dtype=torch.float64, device=dev)
+
+    def k_strain():
+        pat.strain(u, E_tmp)
+
+    def k_spmv():
+        K.matvec(u, y)
+
+    step = [("constitutive", k_const), ("force", k_force), ("tangent", k_tangent)]
+    extra = [("strain", k_strain), ("spmv", k_spmv)]
+    for _ in range(args.warmup):
+        for _, f in step + extra:
+            f()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K steps
+    ev = {name: [] for name, _ in step + extra}
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record()
+    for _ in range(args.steps):
+        for name, f in step:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            f()
+            b.record()
+            ev[name].append((a, b))
+    t_end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    step_ms = t_start.elapsed_time(t_end) / args.steps
+    # side kernels (not part of the step), timed separately
+    for _ in range(args.steps):
+        for name, f in extra:
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record()
+            f()
+            b.record()
+            ev[name].append((a, b))
+    torch.cuda.synchronize()
+    ab = algorithmic_bytes(op)
+    kern = {}
+    for name, lst in ev.items():
+        ms = float(np.mean([a.elapsed_time(b) for a, b in lst]))
+        kern[name] = {"avg_us": round(ms * 1e3, 2), "alg_bytes": int(ab[name]),
+                      "GB_s": round(ab[name] / (ms * 1e-3) / 1e9, 1)}
+    step_bytes = ab["constitutive"] + ab["force"] + ab["tangent"]
+    value = step_bytes / (step_ms * 1e-3) / 1e9
+    dom = max(("constitutive", "force", "tangent"), key=lambda k: kern[k]["avg_us"])
+    # branch histogram of the inputs as the device classified them
+    br_dev = torch.empty(op.n_int, dtype=torch.int8, device=dev)
+    _lib.call("ssx_constitutive_3d", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear), ptr(op.lame),
+              op.n_int, ptr(S), None, None, ptr(br_dev), st)
+    hist = torch.bincount(br_dev.to(torch.int64), minlength=5).cpu().numpy()
+    traffic = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        try:
+            traffic = json.loads(tfile.read_text()).get(dom)
+        except Exception:
+            traffic = None
+    return {
+        "m": m, "op": op, "E": E_host, "step_ms": step_ms, "value": value, "kernels": kern, "dominant": dom,
+        "wall_s": wall, "hist": hist.tolist(), "traffic": traffic, "ab": ab,
+    }
+
+
+def cpu_baseline(op, E_host, ab, budget_s: float):
+    """The reference's own C kernels (compiled from its sources into oracle/_ref) on the host cores:
+    return mapping + force rows + tangent rows over a bounded sample of the same workload."""
+    import oracle
+
+    if oracle.ref() is None:
+        return None
+    geom = op.geom
+    pat = op.pattern.host
+    n_e_total = geom.n_elem
+    # sample: the first n_e elements' points (and the rows they touch are approximated by the full
+    # row lists restricted to those elements); time scales linearly in points, reported per byte
+    n = geom.n_int
+    cb = np.full(n, 4 * np.sqrt(3))
+    sp = np.full(n, 1 / np.sqrt(3))
+    from slope_stability_petsc4py_amd import workloads
+    G = np.full(n, workloads.HOMO.shear)
+    L = np.full(n, workloads.HOMO.lame)
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count()
+    t0 = time.perf_counter()
+    reps = 0
+    t_const = t_force = t_tan = 0.0
+    while True:
+        a = time.perf_counter()
+        S, DS = oracle.ref_constitutive_3d(E_host, cb, sp, G, L)
+        b = time.perf_counter()
+        oracle.ref_force_3d_rows(geom.dphi, S, geom.weight, pat, geom.n_q)
+        c = time.perf_counter()
+        oracle.ref_tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+        d = time.perf_counter()
+        t_const += b - a
+        t_force += c - b
+        t_tan += d - c
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 5:
+            break
+    per = (t_const + t_force + t_tan) / reps
+    step_bytes = ab["constitutive"] + ab["force"] + ab["tangent"]
+    return {
+        "value": round(step_bytes / per / 1e9, 2),
+        "unit": "GB/s",
+        "cores": threads,
+        "kind": "reference",
+        "sample": f"full workload ({n_e_total} tets, {n} points), {reps} reps of return mapping + force rows + "
+                  f"tangent rows (reference C, -O3 -fopenmp); s/step {per:.3f} "
+                  f"(const {t_const / reps:.3f}, force {t_force / reps:.3f}, tangent {t_tan / reps:.3f})",
+        "s_per_step": round(per, 4),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--h", type=float, default=1.0)
+    ap.add_argument("--nz", type=int, default=30)
+    ap.add_argument("--unique", type=int, default=131072)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the boxes here have one GPU); "
+              "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
+        sys.exit(2)
+    res = run_gpu(args)
+    op = res["op"]
+    kern = res["kernels"]
+    dom = res["dominant"]
+    dk = kern[dom]
+    achieved = dk["alg_bytes"] / (dk["avg_us"] * 1e-6) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        _log("cpu baseline (reference C on host cores)...")
+        cpu = cpu_baseline(op, res["E"], res["ab"], args.cpu_budget)
+    cpu_model = ""
+    try:
+        cpu_model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        cpu_model = next((l.split(":", 1)[1].strip() for l in cpu_model.splitlines() if "Model name" in l), "")
+    except Exception:
+        pass
+    out = {
+        "metric": METRIC,
+        "value": round(res["value"], 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(res["step_ms"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3D homogeneous P2 slope kernel bench: return mapping + force + K_r tangent assembly "
+                        "(FP64, FP32 shadow)",
+            "n_elem": int(op.pattern.n_elem), "n_points": int(op.n_int), "n_dof": int(op.pattern.n_rows),
+            "nnz": int(op.pattern.host.nnz), "branch_hist_el_smooth_left_right_apex": res["hist"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": res["traffic"],
+        },
+        "cpu_baseline": cpu,
+        "kernels": kern,
+        "host": {"cpu_model": cpu_model, "nproc": os.cpu_count(), "python": platform.python_version()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,132 @@
+/*
+ * ssx.h -- C ABI of libssx.so, the gfx950 (MI355X) HIP implementation of the slope-stability hot
+ * path: Mohr-Coulomb return mapping, fixed-pattern tangent assembly, CSR operator and the vector
+ * kernels of the Krylov / multigrid solvers.
+ *
+ * Conventions for every entry point
+ *   - pointers are device pointers (HBM), owned by the caller; the library never allocates or
+ *     frees caller data;
+ *   - sizes are int64_t; `stream` is a hipStream_t (NULL = default stream); all work is
+ *     stream-ordered, nothing synchronises unless stated;
+ *   - return 0 on success, 1 for invalid arguments (message via ssx_last_error()), 2 when a
+ *     kernel launch failed, 3 for other runtime errors;
+ *   - point-major layouts: E[p*ns + c], S[p*ns + c], DS[p*ns*ns + i + ns*j] = dS_i/dE_j;
+ *     dphi[(d*n_int + p)*n_p + node]; 3D strain [e11 e22 e33 g12 g23 g13], 2D [e11 e22 g12],
+ *     2D stress [s11 s22 s12 s33]; global dof = dim*node + comp.
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   cython/constitutive_3d_batch.h:4-25, cython/assemble_tangent_values_3d.h:6-68 (declared to
+ *   Python at cython/_kernels.pyx:14-97) and the NumPy/PETSc calls named per entry point below.
+ */
+#ifndef SSX_H
+#define SSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library ABI version. */
+int ssx_version(void);
+/* Message of the last failed call on this host thread. */
+const char *ssx_last_error(void);
+/* hipStreamSynchronize(stream). */
+int ssx_stream_sync(void *stream);
+
+/* Davis strength reduction per point: (c0, phi, psi [rad], lambda) -> (c_bar, sin_phi).
+ * davis_type 0/1/2 = A/B/C.  Replaces constitutive/reduction.py:14-78 (`reduction`). */
+int ssx_reduction(const double *c0, const double *phi, const double *psi, int64_t n, double lam, int davis_type,
+                  double *c_bar, double *sin_phi, void *stream);
+
+/* 3D Mohr-Coulomb return mapping: E (n x 6) -> S (n x 6), DS (n x 36, may be NULL), optional FP32
+ * copy of DS and per-point branch id (0 elastic, 1 smooth, 2 left edge, 3 right edge, 4 apex).
+ * Replaces constitutive_problem_3d_s_batch_c / constitutive_problem_3d_sds_batch_c
+ * (cython/constitutive_3d_batch.h:4-25, cython/constitutive_3d_batch.c:9-50). */
+int ssx_constitutive_3d(const double *E, const double *c_bar, const double *sin_phi, const double *shear,
+                        const double *lame, int64_t n, double *S, double *DS, float *DS32, int8_t *branch,
+                        void *stream);
+
+/* Same with material data as a per-element id into a table of rows (c_bar, sin_phi, shear, lame);
+ * n = n_elem * n_q.  Same reference entry points as ssx_constitutive_3d, with the per-point
+ * material arrays of cython/_kernels.pyx:100-123 folded into a table. */
+int ssx_constitutive_3d_mat(const double *E, const int32_t *mat_id, const double *table, int n_mat, int n_q,
+                            int64_t n, double *S, double *DS, float *DS32, int8_t *branch, void *stream);
+
+/* Plane-strain return mapping: E (n x 3) -> S (n x 4), DS (n x 9).
+ * Replaces constitutive_problem_2D (constitutive/problem.py:143-475). */
+int ssx_constitutive_2d(const double *E, const double *c_bar, const double *sin_phi, const double *shear,
+                        const double *bulk, const double *lame, int64_t n, double *S, double *DS, float *DS32,
+                        int8_t *branch, void *stream);
+
+/* Strain gather E = B u for (dim, n_p) in {(3,4),(3,10),(3,35),(2,3),(2,6),(2,15)}; edofs is
+ * (n_elem x dim*n_p) int32 dof numbers into u.
+ * Replaces assemble_overlap_strain_3d_c (cython/assemble_tangent_values_3d.h:50-61) and the 2D
+ * `overlap_B @ u` (fem/distributed_tangent.py:906-907). */
+int ssx_strain(int dim, int n_p, const double *dphi, const int32_t *edofs, const double *u, int64_t n_elem, int n_q,
+               double *E, void *stream);
+
+/* Internal force, stage 1: element vectors fe (n_elem x dim*n_p) = sum_q w B^T S.
+ * Together with ssx_force_gather replaces assemble_force_3d_rows_c
+ * (cython/assemble_tangent_values_3d.h:63-68) and the 2D B^T(wS) (constitutive/problem.py:1954). */
+int ssx_force_elem(int dim, int n_p, const double *dphi, const double *S, const double *w, int64_t n_elem, int n_q,
+                   double *fe, void *stream);
+
+/* Internal force, stage 2: F[row] = sum over the row's slots of fe[slot_elem*nld + slot_lrow];
+ * rows without slots (constrained) get 0 (fem/distributed_tangent.py:932). */
+int ssx_force_gather(const double *fe, const int32_t *row_slot_ptr, const int32_t *slot_elem, const uint8_t *slot_lrow,
+                     int64_t n_rows, int nld, double *F, void *stream);
+
+/* Tangent values on the fixed CSR pattern through the row-slot lists; constrained rows get a unit
+ * diagonal.  With `elastic` non-NULL writes K_r = (1-r) K + r K_el; `out32` (may be NULL) receives
+ * an FP32 copy.  max_block_nnz = max over 64-row blocks of the block's nnz (sizes the LDS tile).
+ * Replaces assemble_tangent_values_3d_rows_c (cython/assemble_tangent_values_3d.h:21-38) with the
+ * diagonal fix-up of fem/distributed_tangent.py:1606-1607 and the K_r blend of
+ * constitutive/problem.py:1614-1615. */
+int ssx_tangent_rows(int dim, int n_p, const double *dphi, const double *DS, const double *w,
+                     const int32_t *row_slot_ptr, const int32_t *slot_elem, const uint8_t *slot_lrow,
+                     const int32_t *slot_pos, const int32_t *indptr, const uint8_t *row_free, int64_t n_rows,
+                     int64_t n_elem, int n_q, double r, const double *elastic, double *out, float *out32,
+                     int64_t max_block_nnz, void *stream);
+
+/* y = alpha*A x + beta*y for CSR A (int32 indptr/indices); lanes per row: power of two <= 64.
+ * Replaces PETSc MatMult (linear/solver.py:3431-3451). */
+int ssx_spmv_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, const double *x,
+                 double *y, double alpha, double beta, int lanes, void *stream);
+int ssx_spmv_f32(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const float *val, const float *x,
+                 float *y, float alpha, float beta, int lanes, void *stream);
+
+/* y = a*x + b*y (VecAXPBY in linear/deflated_fgmres.py:174-295). */
+int ssx_axpby_f64(int64_t n, double a, const double *x, double b, double *y, void *stream);
+int ssx_axpby_f32(int64_t n, float a, const float *x, float b, float *y, void *stream);
+/* Precision casts at the FP32 preconditioner boundary. */
+int ssx_cast_f64_f32(int64_t n, const double *x, float *y, void *stream);
+int ssx_cast_f32_f64(int64_t n, const float *x, double *y, void *stream);
+
+/* out[j] = V[:, j] . x for j < k (column stride ld): the classical Gram-Schmidt projections of one
+ * Arnoldi step (linear/deflated_fgmres.py:261-263, cython/_kernels.pyx:190-192). */
+int ssx_dot_cols(int64_t n, int k, const double *V, int64_t ld, const double *x, double *out, void *stream);
+/* out[0] = x . y, deterministic two-stage reduction; part: >= 1024 doubles of scratch
+ * (cython/_kernels.pyx:100-123 `dot`/`norm2`). */
+int ssx_dot(int64_t n, const double *x, const double *y, double *part, double *out, void *stream);
+/* x += sign * V[:, :k] h (the CGS update and the Krylov solution update). */
+int ssx_gemv_cols(int64_t n, int k, const double *V, int64_t ld, const double *h, double sign, double *x,
+                  void *stream);
+
+/* Chebyshev/Jacobi smoother step: d = a*d + c*dinv.*r; x += d (replaces the Richardson/SOR and
+ * Chebyshev/Jacobi smoother applies of linear/solver.py:155-162,417-436). */
+int ssx_cheb_update_f32(int64_t n, const float *dinv, const float *r, float c, float a, float *x, float *d,
+                        void *stream);
+int ssx_cheb_update_f64(int64_t n, const double *dinv, const double *r, double c, double a, double *x, double *d,
+                        void *stream);
+/* dinv[r] = 1 / A[r, r + col0] (0 where the diagonal is zero). */
+int ssx_inv_diag_f32(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const float *val,
+                     float *dinv, void *stream);
+int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const double *val,
+                     double *dinv, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SSX_H */

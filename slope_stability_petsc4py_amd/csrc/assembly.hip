@@ -1,0 +1,346 @@
+// Fixed-pattern finite-element assembly on gfx950: strain gather E = B u, internal force
+// F = B^T (w S), and tangent values K = sum B^T D B written into the fixed CSR pattern, with the
+// regularised operator K_r = (1-r) K + r K_el and an optional FP32 copy produced in the same pass.
+//
+// Reference behaviour: cython/assemble_tangent_values_3d.c:216-273 (strain), :275-327 (force),
+// :111-214 (tangent rows); fem/distributed_tangent.py:1559-1612 (constrained diagonal = 1),
+// constitutive/problem.py:1595-1644 (K_r on the same pattern).
+//
+// HBM layout: dphi is (dim, n_int, n_p) (one (n_int, n_p) slab per spatial derivative, the
+// reference's dphi1/dphi2/dphi3), w is (n_int), E/S are point-major (n_int, n_strain), DS is
+// point-major (n_int, n_strain^2, column-major per point).  Element DOF tables are (n_elem, dim*n_p).
+#include "ssx_common.h"
+
+namespace {
+
+// Strain-displacement column for local dof (node j, comp d): the nonzero strain rows.
+// 3D strain order [e11 e22 e33 g12 g23 g13]; 2D [e11 e22 g12].
+template <int DIM>
+struct BCol;
+
+template <>
+struct BCol<3> {
+    static constexpr int NS = 6;
+    // B^T s for component d of a node with gradient (gx, gy, gz)
+    __device__ static __forceinline__ double bts(int d, double gx, double gy, double gz, const double *s) {
+        if (d == 0) return gx * s[0] + gy * s[3] + gz * s[5];
+        if (d == 1) return gx * s[3] + gy * s[1] + gz * s[4];
+        return gx * s[5] + gy * s[4] + gz * s[2];
+    }
+};
+
+template <>
+struct BCol<2> {
+    static constexpr int NS = 3;
+    __device__ static __forceinline__ double bts(int d, double gx, double gy, double, const double *s) {
+        if (d == 0) return gx * s[0] + gy * s[2];
+        return gy * s[1] + gx * s[2];
+    }
+};
+
+// ------------------------------------------------------------------------------------------
+// Strain gather: one lane per quadrature point.
+// ------------------------------------------------------------------------------------------
+template <int DIM, int NP>
+__global__ void __launch_bounds__(256) k_strain(const double *__restrict__ dphi, const int32_t *__restrict__ edofs,
+                                                const double *__restrict__ u, int64_t n_int, int n_q,
+                                                double *__restrict__ E) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n_int) return;
+    const int64_t e = g / n_q;
+    const int32_t *dof = edofs + e * (DIM * NP);
+    const double *gx = dphi + g * NP;
+    const double *gy = dphi + (n_int + g) * NP;
+    if constexpr (DIM == 3) {
+        const double *gz = dphi + (2 * n_int + g) * NP;
+        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const double ux = u[dof[3 * j]], uy = u[dof[3 * j + 1]], uz = u[dof[3 * j + 2]];
+            const double x = gx[j], y = gy[j], z = gz[j];
+            a0 += x * ux;
+            a1 += y * uy;
+            a2 += z * uz;
+            a3 += y * ux + x * uy;
+            a4 += z * uy + y * uz;
+            a5 += z * ux + x * uz;
+        }
+        double *o = E + g * 6;
+        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4; o[5] = a5;
+    } else {
+        double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+        for (int j = 0; j < NP; ++j) {
+            const double ux = u[dof[2 * j]], uy = u[dof[2 * j + 1]];
+            const double x = gx[j], y = gy[j];
+            a0 += x * ux;
+            a1 += y * uy;
+            a2 += y * ux + x * uy;
+        }
+        double *o = E + g * 3;
+        o[0] = a0; o[1] = a1; o[2] = a2;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Internal force, stage 1: element-local vector f_e[a] = sum_q w_q B_a(q)^T S_q, one lane per
+// (element, local dof).  Stage 2 gathers f_e into rows through the row-slot lists, so every row
+// is written by one lane in a fixed (element) order: deterministic, no atomics.
+// ------------------------------------------------------------------------------------------
+template <int DIM, int NP>
+__global__ void __launch_bounds__(256) k_force_elem(const double *__restrict__ dphi, const double *__restrict__ S,
+                                                    const double *__restrict__ w, int64_t n_elem, int64_t n_int,
+                                                    int n_q, double *__restrict__ fe) {
+    constexpr int NLD = DIM * NP;
+    constexpr int NS = BCol<DIM>::NS;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n_elem * NLD) return;
+    const int64_t e = t / NLD;
+    const int a = (int)(t - e * NLD);
+    const int node = a / DIM, comp = a % DIM;
+    double acc = 0.0;
+    for (int q = 0; q < n_q; ++q) {
+        const int64_t g = e * n_q + q;
+        const double gx = dphi[g * NP + node];
+        const double gy = dphi[(n_int + g) * NP + node];
+        const double gz = DIM == 3 ? dphi[(2 * n_int + g) * NP + node] : 0.0;
+        acc += w[g] * BCol<DIM>::bts(comp, gx, gy, gz, S + g * NS);
+    }
+    fe[t] = acc;
+}
+
+__global__ void __launch_bounds__(256) k_force_gather(const double *__restrict__ fe, const int32_t *__restrict__ row_slot_ptr,
+                                                      const int32_t *__restrict__ slot_elem,
+                                                      const uint8_t *__restrict__ slot_lrow, int64_t n_rows, int nld,
+                                                      double *__restrict__ F) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    double acc = 0.0;
+    const int32_t s1 = row_slot_ptr[r + 1];
+    for (int32_t s = row_slot_ptr[r]; s < s1; ++s) acc += fe[(int64_t)slot_elem[s] * nld + slot_lrow[s]];
+    F[r] = acc;  // rows without slots (constrained) get 0, as the reference zeroes them
+}
+
+// ------------------------------------------------------------------------------------------
+// Tangent values on the fixed pattern.  One workgroup of ROWS lanes owns ROWS consecutive rows,
+// i.e. one contiguous run of the CSR value array; with USE_LDS that run is accumulated in LDS and
+// written once, fully coalesced, with the regularisation and the FP32 copy fused in.  Each lane
+// walks its row's slots in element order; per slot it sums the point contributions of its
+// 3*n_p (or 2*n_p) columns in registers and adds them to the row once.
+// ------------------------------------------------------------------------------------------
+constexpr int kRows = 64;
+
+template <int DIM, int NP, bool USE_LDS>
+__global__ void __launch_bounds__(kRows) k_tangent_rows(
+    const double *__restrict__ dphi, const double *__restrict__ DS, const double *__restrict__ w,
+    const int32_t *__restrict__ row_slot_ptr, const int32_t *__restrict__ slot_elem,
+    const uint8_t *__restrict__ slot_lrow, const int32_t *__restrict__ slot_pos, const int32_t *__restrict__ indptr,
+    const uint8_t *__restrict__ row_free, int64_t n_rows, int64_t n_int, int n_q, double r,
+    const double *__restrict__ elastic, double *__restrict__ out, float *__restrict__ out32) {
+    constexpr int NLD = DIM * NP;
+    constexpr int NS = BCol<DIM>::NS;
+    extern __shared__ double lds[];
+    const int64_t r0 = (int64_t)blockIdx.x * kRows;
+    const int64_t r1 = r0 + kRows < n_rows ? r0 + kRows : n_rows;
+    const int64_t v0 = indptr[r0];
+    const int64_t L = (int64_t)indptr[r1] - v0;
+    double *buf = USE_LDS ? lds : out + v0;
+    for (int64_t i = threadIdx.x; i < L; i += kRows) buf[i] = 0.0;
+    __syncthreads();
+
+    const int64_t row = r0 + threadIdx.x;
+    if (row < r1) {
+        if (!row_free[row]) {
+            buf[indptr[row] - v0] = 1.0;  // constrained row: only its diagonal, set to 1
+        } else {
+            const int32_t s_end = row_slot_ptr[row + 1];
+            for (int32_t s = row_slot_ptr[row]; s < s_end; ++s) {
+                const int64_t e = slot_elem[s];
+                const int a = slot_lrow[s];
+                const int ni = a / DIM, ci = a % DIM;
+                double acc[NLD];
+#pragma unroll
+                for (int b = 0; b < NLD; ++b) acc[b] = 0.0;
+                for (int q = 0; q < n_q; ++q) {
+                    const int64_t g = e * n_q + q;
+                    const double *gx = dphi + g * NP;
+                    const double *gy = dphi + (n_int + g) * NP;
+                    const double *gz = DIM == 3 ? dphi + (2 * n_int + g) * NP : gx;
+                    const double *D = DS + g * (NS * NS);
+                    const double wg = w[g];
+                    // t_m = w * sum_s B_row[s] D[s, m]  (D column-major: D[s + NS*m])
+                    double t[NS];
+                    if constexpr (DIM == 3) {
+                        const double xi = gx[ni], yi = gy[ni], zi = gz[ni];
+#pragma unroll
+                        for (int m = 0; m < 6; ++m) {
+                            const double *Dm = D + 6 * m;
+                            double v;
+                            if (ci == 0) v = xi * Dm[0] + yi * Dm[3] + zi * Dm[5];
+                            else if (ci == 1) v = yi * Dm[1] + xi * Dm[3] + zi * Dm[4];
+                            else v = zi * Dm[2] + yi * Dm[4] + xi * Dm[5];
+                            t[m] = wg * v;
+                        }
+#pragma unroll
+                        for (int j = 0; j < NP; ++j) {
+                            const double xj = gx[j], yj = gy[j], zj = gz[j];
+                            acc[3 * j + 0] += t[0] * xj + t[3] * yj + t[5] * zj;
+                            acc[3 * j + 1] += t[3] * xj + t[1] * yj + t[4] * zj;
+                            acc[3 * j + 2] += t[5] * xj + t[4] * yj + t[2] * zj;
+                        }
+                    } else {
+                        const double xi = gx[ni], yi = gy[ni];
+#pragma unroll
+                        for (int m = 0; m < 3; ++m) {
+                            const double *Dm = D + 3 * m;
+                            const double v = ci == 0 ? xi * Dm[0] + yi * Dm[2] : yi * Dm[1] + xi * Dm[2];
+                            t[m] = wg * v;
+                        }
+#pragma unroll
+                        for (int j = 0; j < NP; ++j) {
+                            const double xj = gx[j], yj = gy[j];
+                            acc[2 * j + 0] += t[0] * xj + t[2] * yj;
+                            acc[2 * j + 1] += t[2] * xj + t[1] * yj;
+                        }
+                    }
+                }
+                const int32_t *pos = slot_pos + (int64_t)s * NLD;
+#pragma unroll
+                for (int b = 0; b < NLD; ++b) {
+                    const int32_t p = pos[b];
+                    if (p >= 0) buf[p - v0] += acc[b];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const double one_minus_r = 1.0 - r;
+    for (int64_t i = threadIdx.x; i < L; i += kRows) {
+        double v = buf[i];
+        if (elastic) v = one_minus_r * v + r * elastic[v0 + i];
+        if (USE_LDS) out[v0 + i] = v;
+        else buf[i] = v;
+        if (out32) out32[v0 + i] = (float)v;
+    }
+}
+
+template <int DIM, int NP>
+int launch_tangent(const double *dphi, const double *DS, const double *w, const int32_t *row_slot_ptr,
+                   const int32_t *slot_elem, const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr,
+                   const uint8_t *row_free, int64_t n_rows, int64_t n_int, int n_q, double r, const double *elastic,
+                   double *out, float *out32, int64_t max_block_nnz, hipStream_t st) {
+    const unsigned grid = ssx::grid_for(n_rows, kRows);
+    const size_t lds_bytes = (size_t)max_block_nnz * sizeof(double);
+    static int max_lds = -1;
+    if (max_lds < 0) {
+        int dev = 0, v = 65536;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        max_lds = v;
+    }
+    bool lds_ok = max_block_nnz >= 0 && lds_bytes <= (size_t)max_lds;
+    if (lds_ok && lds_bytes > 65536)
+        lds_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_tangent_rows<DIM, NP, true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
+    if (lds_ok) {
+        hipLaunchKernelGGL((k_tangent_rows<DIM, NP, true>), dim3(grid), dim3(kRows), lds_bytes, st, dphi, DS, w,
+                           row_slot_ptr, slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, n_int, n_q, r,
+                           elastic, out, out32);
+    } else {
+        hipLaunchKernelGGL((k_tangent_rows<DIM, NP, false>), dim3(grid), dim3(kRows), 0, st, dphi, DS, w, row_slot_ptr,
+                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, n_int, n_q, r, elastic, out,
+                           out32);
+    }
+    SSX_CHECK_LAUNCH("ssx_tangent_rows");
+    return SSX_OK;
+}
+
+template <int DIM, int NP>
+int launch_strain(const double *dphi, const int32_t *edofs, const double *u, int64_t n_int, int n_q, double *E,
+                  hipStream_t st) {
+    hipLaunchKernelGGL((k_strain<DIM, NP>), dim3(ssx::grid_for(n_int, 256)), dim3(256), 0, st, dphi, edofs, u, n_int,
+                       n_q, E);
+    SSX_CHECK_LAUNCH("ssx_strain");
+    return SSX_OK;
+}
+
+template <int DIM, int NP>
+int launch_force_elem(const double *dphi, const double *S, const double *w, int64_t n_elem, int64_t n_int, int n_q,
+                      double *fe, hipStream_t st) {
+    hipLaunchKernelGGL((k_force_elem<DIM, NP>), dim3(ssx::grid_for(n_elem * DIM * NP, 256)), dim3(256), 0, st, dphi,
+                       S, w, n_elem, n_int, n_q, fe);
+    SSX_CHECK_LAUNCH("ssx_force");
+    return SSX_OK;
+}
+
+}  // namespace
+
+// Dispatch over the supported (dim, n_p) pairs: 3D P1/P2/P4 tetrahedra, 2D P1/P2/P4 triangles.
+#define SSX_DISPATCH(dim, n_p, CALL)                                                           \
+    do {                                                                                       \
+        if (dim == 3 && n_p == 4) return CALL(3, 4);                                           \
+        if (dim == 3 && n_p == 10) return CALL(3, 10);                                         \
+        if (dim == 3 && n_p == 35) return CALL(3, 35);                                         \
+        if (dim == 2 && n_p == 3) return CALL(2, 3);                                           \
+        if (dim == 2 && n_p == 6) return CALL(2, 6);                                           \
+        if (dim == 2 && n_p == 15) return CALL(2, 15);                                         \
+        ssx::set_error("unsupported element: dim=%d n_p=%d", dim, n_p);                        \
+        return SSX_EINVAL;                                                                     \
+    } while (0)
+
+extern "C" {
+
+int ssx_strain(int dim, int n_p, const double *dphi, const int32_t *edofs, const double *u, int64_t n_elem, int n_q,
+               double *E, void *stream) {
+    SSX_REQUIRE(n_elem >= 0 && n_q > 0, "ssx_strain: bad sizes");
+    SSX_REQUIRE(dphi && edofs && u && E, "ssx_strain: null pointer");
+    if (n_elem == 0) return SSX_OK;
+    const int64_t n_int = n_elem * n_q;
+    hipStream_t st = ssx::as_stream(stream);
+#define CALL(D, P) launch_strain<D, P>(dphi, edofs, u, n_int, n_q, E, st)
+    SSX_DISPATCH(dim, n_p, CALL);
+#undef CALL
+}
+
+int ssx_force_elem(int dim, int n_p, const double *dphi, const double *S, const double *w, int64_t n_elem, int n_q,
+                   double *fe, void *stream) {
+    SSX_REQUIRE(n_elem >= 0 && n_q > 0, "ssx_force_elem: bad sizes");
+    SSX_REQUIRE(dphi && S && w && fe, "ssx_force_elem: null pointer");
+    if (n_elem == 0) return SSX_OK;
+    const int64_t n_int = n_elem * n_q;
+    hipStream_t st = ssx::as_stream(stream);
+#define CALL(D, P) launch_force_elem<D, P>(dphi, S, w, n_elem, n_int, n_q, fe, st)
+    SSX_DISPATCH(dim, n_p, CALL);
+#undef CALL
+}
+
+int ssx_force_gather(const double *fe, const int32_t *row_slot_ptr, const int32_t *slot_elem, const uint8_t *slot_lrow,
+                     int64_t n_rows, int nld, double *F, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && nld > 0, "ssx_force_gather: bad sizes");
+    if (n_rows == 0) return SSX_OK;
+    SSX_REQUIRE(fe && row_slot_ptr && F, "ssx_force_gather: null pointer");
+    hipLaunchKernelGGL(k_force_gather, dim3(ssx::grid_for(n_rows, 256)), dim3(256), 0, ssx::as_stream(stream), fe,
+                       row_slot_ptr, slot_elem, slot_lrow, n_rows, nld, F);
+    SSX_CHECK_LAUNCH("ssx_force_gather");
+    return SSX_OK;
+}
+
+int ssx_tangent_rows(int dim, int n_p, const double *dphi, const double *DS, const double *w,
+                     const int32_t *row_slot_ptr, const int32_t *slot_elem, const uint8_t *slot_lrow,
+                     const int32_t *slot_pos, const int32_t *indptr, const uint8_t *row_free, int64_t n_rows,
+                     int64_t n_elem, int n_q, double r, const double *elastic, double *out, float *out32,
+                     int64_t max_block_nnz, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && n_elem >= 0 && n_q > 0, "ssx_tangent_rows: bad sizes");
+    if (n_rows == 0) return SSX_OK;
+    SSX_REQUIRE(dphi && DS && w && row_slot_ptr && indptr && row_free && out, "ssx_tangent_rows: null pointer");
+    const int64_t n_int = n_elem * n_q;
+    hipStream_t st = ssx::as_stream(stream);
+#define CALL(D, P)                                                                                                 \
+    launch_tangent<D, P>(dphi, DS, w, row_slot_ptr, slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, n_int, \
+                         n_q, r, elastic, out, out32, max_block_nnz, st)
+    SSX_DISPATCH(dim, n_p, CALL);
+#undef CALL
+}
+
+}  // extern "C"

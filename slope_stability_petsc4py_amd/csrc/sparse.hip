@@ -1,0 +1,260 @@
+// CSR operator applies and the vector kernels of the Krylov/multigrid solvers on gfx950.
+//
+// Reference behaviour: the PETSc MatMult / VecDot / VecAXPY calls of linear/solver.py:3431-3451 and
+// linear/deflated_fgmres.py:174-295 (classical Gram-Schmidt over the Arnoldi basis), and the
+// smoother applies of linear/solver.py:155-162,417-436.
+//
+// SpMV: a group of G lanes per row (G a power of two picked by the host from the mean row
+// length), lanes stride the row's nonzeros so each load instruction of the group reads G
+// consecutive values/indices; partial sums are combined with cross-lane shuffles.  FP64 values for
+// the Newton operator, FP32 values for the preconditioner; the vector precision matches the values.
+#include "ssx_common.h"
+
+namespace {
+
+template <typename T, int G>
+__global__ void __launch_bounds__(256) k_spmv(int64_t n_rows, const int32_t *__restrict__ indptr,
+                                              const int32_t *__restrict__ indices, const T *__restrict__ val,
+                                              const T *__restrict__ x, T *__restrict__ y, T alpha, T beta) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = tid / G;
+    const int lane = (int)(tid % G);
+    T acc = 0;
+    if (row < n_rows) {
+        const int32_t k1 = indptr[row + 1];
+        for (int32_t k = indptr[row] + lane; k < k1; k += G) acc += val[k] * x[indices[k]];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, G);
+    if (row < n_rows && lane == 0) y[row] = beta == (T)0 ? alpha * acc : alpha * acc + beta * y[row];
+}
+
+template <typename T>
+int launch_spmv(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const T *val, const T *x, T *y,
+                T alpha, T beta, int lanes, hipStream_t st) {
+    const int64_t threads = n_rows * lanes;
+    const dim3 grid(ssx::grid_for(threads, 256)), block(256);
+    switch (lanes) {
+        case 1: hipLaunchKernelGGL((k_spmv<T, 1>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        case 2: hipLaunchKernelGGL((k_spmv<T, 2>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        case 4: hipLaunchKernelGGL((k_spmv<T, 4>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        case 8: hipLaunchKernelGGL((k_spmv<T, 8>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        case 16: hipLaunchKernelGGL((k_spmv<T, 16>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        case 32: hipLaunchKernelGGL((k_spmv<T, 32>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        case 64: hipLaunchKernelGGL((k_spmv<T, 64>), grid, block, 0, st, n_rows, indptr, indices, val, x, y, alpha, beta); break;
+        default: ssx::set_error("ssx_spmv: lanes per row must be a power of two <= 64"); return SSX_EINVAL;
+    }
+    SSX_CHECK_LAUNCH("ssx_spmv");
+    return SSX_OK;
+}
+
+// y = a*x + b*y
+template <typename T>
+__global__ void k_axpby(int64_t n, T a, const T *__restrict__ x, T b, T *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = b == (T)0 ? a * x[i] : a * x[i] + b * y[i];
+}
+
+// Precision casts for vectors (FP64 <-> FP32) at the preconditioner boundary.
+__global__ void k_d2f(int64_t n, const double *__restrict__ x, float *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (float)x[i];
+}
+__global__ void k_f2d(int64_t n, const float *__restrict__ x, double *__restrict__ y) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = (double)x[i];
+}
+
+// out[j] = sum_i V[j*ld + i] * x[i] for j < k: one workgroup per column j, deterministic tree
+// reduction (the classical Gram-Schmidt projections of one Arnoldi step in a single launch).
+__global__ void __launch_bounds__(256) k_dot_cols(int64_t n, int k, const double *__restrict__ V, int64_t ld,
+                                                  const double *__restrict__ x, double *__restrict__ out) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    const double *v = V + (int64_t)j * ld;
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += v[i] * x[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+}
+
+// Two-stage deterministic dot for long vectors: partials per workgroup, then one final sum.
+__global__ void __launch_bounds__(256) k_dot_partial(int64_t n, const double *__restrict__ x,
+                                                     const double *__restrict__ y, double *__restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        acc += x[i] * y[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) k_sum(int n, const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) acc += part[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// x -= V h  (x[i] -= sum_j V[j*ld+i] h[j]); and x += V h with sign = +1.
+__global__ void k_gemv_cols(int64_t n, int k, const double *__restrict__ V, int64_t ld, const double *__restrict__ h,
+                            double sign, double *__restrict__ x) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (int j = 0; j < k; ++j) acc += V[(int64_t)j * ld + i] * h[j];
+    x[i] += sign * acc;
+}
+
+// One damped-Jacobi / Chebyshev step kernel: x_new = x + a*(x - x_prev) + c * dinv .* (b - A x)
+// with the residual r = b - A x supplied; writes the new iterate and keeps the previous one.
+template <typename T>
+__global__ void k_cheb_update(int64_t n, const T *__restrict__ dinv, const T *__restrict__ r, T c, T a,
+                              T *__restrict__ x, T *__restrict__ d) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const T di = a * d[i] + c * dinv[i] * r[i];
+    d[i] = di;
+    x[i] += di;
+}
+
+// Extract the inverse diagonal of a CSR matrix (rows with a zero diagonal get 0).
+template <typename T>
+__global__ void k_inv_diag(int64_t n_rows, int64_t col0, const int32_t *__restrict__ indptr,
+                           const int32_t *__restrict__ indices, const T *__restrict__ val, T *__restrict__ dinv) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_rows) return;
+    T d = 0;
+    for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
+        if ((int64_t)indices[k] == r + col0) d = val[k];
+    dinv[r] = d != (T)0 ? (T)1 / d : (T)0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssx_spmv_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, const double *x,
+                 double *y, double alpha, double beta, int lanes, void *stream) {
+    SSX_REQUIRE(n_rows >= 0, "ssx_spmv_f64: n_rows < 0");
+    if (n_rows == 0) return SSX_OK;
+    return launch_spmv<double>(n_rows, indptr, indices, val, x, y, alpha, beta, lanes, ssx::as_stream(stream));
+}
+
+int ssx_spmv_f32(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const float *val, const float *x,
+                 float *y, float alpha, float beta, int lanes, void *stream) {
+    SSX_REQUIRE(n_rows >= 0, "ssx_spmv_f32: n_rows < 0");
+    if (n_rows == 0) return SSX_OK;
+    return launch_spmv<float>(n_rows, indptr, indices, val, x, y, alpha, beta, lanes, ssx::as_stream(stream));
+}
+
+int ssx_axpby_f64(int64_t n, double a, const double *x, double b, double *y, void *stream) {
+    if (n <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_axpby<double>, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n, a, x, b, y);
+    SSX_CHECK_LAUNCH("ssx_axpby_f64");
+    return SSX_OK;
+}
+
+int ssx_axpby_f32(int64_t n, float a, const float *x, float b, float *y, void *stream) {
+    if (n <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_axpby<float>, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n, a, x, b, y);
+    SSX_CHECK_LAUNCH("ssx_axpby_f32");
+    return SSX_OK;
+}
+
+int ssx_cast_f64_f32(int64_t n, const double *x, float *y, void *stream) {
+    if (n <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_d2f, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n, x, y);
+    SSX_CHECK_LAUNCH("ssx_cast_f64_f32");
+    return SSX_OK;
+}
+
+int ssx_cast_f32_f64(int64_t n, const float *x, double *y, void *stream) {
+    if (n <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_f2d, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n, x, y);
+    SSX_CHECK_LAUNCH("ssx_cast_f32_f64");
+    return SSX_OK;
+}
+
+int ssx_dot_cols(int64_t n, int k, const double *V, int64_t ld, const double *x, double *out, void *stream) {
+    SSX_REQUIRE(n >= 0 && k >= 0 && ld >= n, "ssx_dot_cols: bad sizes");
+    if (k == 0) return SSX_OK;
+    hipLaunchKernelGGL(k_dot_cols, dim3(k), dim3(256), 0, ssx::as_stream(stream), n, k, V, ld, x, out);
+    SSX_CHECK_LAUNCH("ssx_dot_cols");
+    return SSX_OK;
+}
+
+// out[0] = x . y ; `part` is caller scratch of at least 1024 doubles.
+int ssx_dot(int64_t n, const double *x, const double *y, double *part, double *out, void *stream) {
+    SSX_REQUIRE(n >= 0, "ssx_dot: n < 0");
+    const int nb = (int)(n / 256 + 1 < 1024 ? n / 256 + 1 : 1024);
+    hipStream_t st = ssx::as_stream(stream);
+    hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(256), 0, st, n, x, y, part);
+    hipLaunchKernelGGL(k_sum, dim3(1), dim3(256), 0, st, nb, part, out);
+    SSX_CHECK_LAUNCH("ssx_dot");
+    return SSX_OK;
+}
+
+int ssx_gemv_cols(int64_t n, int k, const double *V, int64_t ld, const double *h, double sign, double *x,
+                  void *stream) {
+    SSX_REQUIRE(n >= 0 && k >= 0 && ld >= n, "ssx_gemv_cols: bad sizes");
+    if (n == 0 || k == 0) return SSX_OK;
+    hipLaunchKernelGGL(k_gemv_cols, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n, k, V, ld, h,
+                       sign, x);
+    SSX_CHECK_LAUNCH("ssx_gemv_cols");
+    return SSX_OK;
+}
+
+int ssx_cheb_update_f32(int64_t n, const float *dinv, const float *r, float c, float a, float *x, float *d,
+                        void *stream) {
+    if (n <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_cheb_update<float>, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n, dinv,
+                       r, c, a, x, d);
+    SSX_CHECK_LAUNCH("ssx_cheb_update_f32");
+    return SSX_OK;
+}
+
+int ssx_cheb_update_f64(int64_t n, const double *dinv, const double *r, double c, double a, double *x, double *d,
+                        void *stream) {
+    if (n <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_cheb_update<double>, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), n,
+                       dinv, r, c, a, x, d);
+    SSX_CHECK_LAUNCH("ssx_cheb_update_f64");
+    return SSX_OK;
+}
+
+int ssx_inv_diag_f32(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const float *val,
+                     float *dinv, void *stream) {
+    if (n_rows <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_inv_diag<float>, dim3(ssx::grid_for(n_rows, 256)), dim3(256), 0, ssx::as_stream(stream),
+                       n_rows, col0, indptr, indices, val, dinv);
+    SSX_CHECK_LAUNCH("ssx_inv_diag_f32");
+    return SSX_OK;
+}
+
+int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const double *val,
+                     double *dinv, void *stream) {
+    if (n_rows <= 0) return SSX_OK;
+    hipLaunchKernelGGL(k_inv_diag<double>, dim3(ssx::grid_for(n_rows, 256)), dim3(256), 0, ssx::as_stream(stream),
+                       n_rows, col0, indptr, indices, val, dinv);
+    SSX_CHECK_LAUNCH("ssx_inv_diag_f64");
+    return SSX_OK;
+}
+
+}  // extern "C"

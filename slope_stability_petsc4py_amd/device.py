@@ -1,0 +1,147 @@
+"""Device buffers: torch-ROCm tensors used purely as HBM storage for the ssx kernels, the device
+copy of the fixed pattern, and the device CSR matrix handle returned in place of a PETSc AIJ.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from .fem import Geometry
+from .pattern import OwnedPattern
+
+
+def device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise _lib.SSXError("no GPU visible: the ssx path runs only on a HIP device (no CPU fallback)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream_ptr() -> int:
+    return int(torch.cuda.current_stream().cuda_stream)
+
+
+def to_dev(a, dtype=None) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        t = a
+        if dtype is not None and t.dtype != dtype:
+            t = t.to(dtype)
+        return t.to(device()).contiguous()
+    arr = np.ascontiguousarray(a)
+    t = torch.from_numpy(arr)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.to(device())
+
+
+def ptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else int(t.data_ptr())
+
+
+def spmv_lanes(indptr: np.ndarray) -> int:
+    """Lanes per row for the vector CSR kernel: the power of two nearest the mean row length / 4."""
+    n = max(int(indptr.size) - 1, 1)
+    mean = float(indptr[-1]) / n
+    lanes = 1
+    while lanes < 64 and lanes * 4 < mean:
+        lanes *= 2
+    return lanes
+
+
+@dataclass
+class DeviceCSR:
+    """Square CSR matrix on the device; ``values`` (FP64) is refreshed in place, ``values32`` is the
+    FP32 shadow fed to the preconditioner."""
+
+    n_rows: int
+    indptr: torch.Tensor     # int32
+    indices: torch.Tensor    # int32
+    values: torch.Tensor     # float64
+    values32: torch.Tensor | None
+    lanes: int
+
+    @property
+    def nnz(self) -> int:
+        return int(self.indices.numel())
+
+    @property
+    def shape(self) -> tuple[int, int]:
+        return (self.n_rows, self.n_rows)
+
+    def matvec(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha: float = 1.0, beta: float = 0.0):
+        if y is None:
+            y = torch.empty(self.n_rows, dtype=torch.float64, device=x.device)
+        _lib.call("ssx_spmv_f64", self.n_rows, ptr(self.indptr), ptr(self.indices), ptr(self.values), ptr(x), ptr(y),
+                  float(alpha), float(beta), self.lanes, stream_ptr())
+        return y
+
+    def matvec32(self, x: torch.Tensor, y: torch.Tensor | None = None, alpha: float = 1.0, beta: float = 0.0):
+        if self.values32 is None:
+            raise ValueError("matrix has no FP32 shadow")
+        if y is None:
+            y = torch.empty(self.n_rows, dtype=torch.float32, device=x.device)
+        _lib.call("ssx_spmv_f32", self.n_rows, ptr(self.indptr), ptr(self.indices), ptr(self.values32), ptr(x),
+                  ptr(y), float(alpha), float(beta), self.lanes, stream_ptr())
+        return y
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.values.cpu().numpy(), self.indices.cpu().numpy(), self.indptr.cpu().numpy()),
+                             shape=self.shape)
+
+
+class DevicePattern:
+    """The fixed pattern, row slots and per-point geometry resident in HBM."""
+
+    def __init__(self, pat: OwnedPattern, geom: Geometry):
+        self.host = pat
+        self.geom = geom
+        self.dim = pat.dim
+        self.n_p = pat.n_p
+        self.n_q = geom.n_q
+        self.n_elem = geom.n_elem
+        self.n_int = geom.n_int
+        self.n_rows = pat.n_rows
+        self.nld = pat.dim * pat.n_p
+        self.n_strain = 6 if pat.dim == 3 else 3
+        self.indptr = to_dev(pat.indptr)
+        self.indices = to_dev(pat.indices)
+        self.row_free = to_dev(pat.row_free)
+        self.row_slot_ptr = to_dev(pat.row_slot_ptr)
+        self.slot_elem = to_dev(pat.slot_elem)
+        self.slot_lrow = to_dev(pat.slot_lrow)
+        self.slot_pos = to_dev(pat.slot_pos)
+        self.elem_dofs = to_dev(pat.elem_dofs)
+        self.free_idx = to_dev(pat.free_idx)
+        self.dphi = to_dev(geom.dphi)
+        self.weight = to_dev(geom.weight)
+        self.max_block_nnz = pat.max_block_nnz(64)
+        self.lanes = spmv_lanes(pat.indptr)
+
+    def new_csr(self, with32: bool = True) -> DeviceCSR:
+        dev = self.indptr.device
+        return DeviceCSR(self.n_rows, self.indptr, self.indices,
+                         torch.zeros(self.host.nnz, dtype=torch.float64, device=dev),
+                         torch.zeros(self.host.nnz, dtype=torch.float32, device=dev) if with32 else None,
+                         self.lanes)
+
+    # -- kernels -----------------------------------------------------------------------------
+    def strain(self, u: torch.Tensor, E: torch.Tensor) -> None:
+        _lib.call("ssx_strain", self.dim, self.n_p, ptr(self.dphi), ptr(self.elem_dofs), ptr(u), self.n_elem,
+                  self.n_q, ptr(E), stream_ptr())
+
+    def force(self, S: torch.Tensor, fe: torch.Tensor, F: torch.Tensor) -> None:
+        _lib.call("ssx_force_elem", self.dim, self.n_p, ptr(self.dphi), ptr(S), ptr(self.weight), self.n_elem,
+                  self.n_q, ptr(fe), stream_ptr())
+        _lib.call("ssx_force_gather", ptr(fe), ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow),
+                  self.n_rows, self.nld, ptr(F), stream_ptr())
+
+    def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
+                elastic: torch.Tensor | None = None) -> None:
+        _lib.call("ssx_tangent_rows", self.dim, self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight),
+                  ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow), ptr(self.slot_pos),
+                  ptr(self.indptr), ptr(self.row_free), self.n_rows, self.n_elem, self.n_q, float(r),
+                  ptr(elastic), ptr(out), ptr(out32), self.max_block_nnz, stream_ptr())

@@ -1,0 +1,97 @@
+"""Geometry and mesh setup: quadrature exactness, nodal bases, volumes, the reference's 2D slope
+counts, tet10 elevation order, xyz ordering."""
+
+import math
+from itertools import product
+
+import numpy as np
+import pytest
+
+from slope_stability_petsc4py_amd import fem, mesh
+
+
+def _mono_tet(a, b, c):
+    # integral of x^a y^b z^c over the unit tetrahedron
+    return math.factorial(a) * math.factorial(b) * math.factorial(c) / math.factorial(a + b + c + 3)
+
+
+@pytest.mark.parametrize("et,deg", [("P1", 1), ("P2", 3), ("P4", 6)])
+def test_tet_quadrature_exact(et, deg):
+    xi, wf = fem.quadrature_3d(et)
+    # the P2 rule's published decimals carry ~1e-13 truncation
+    tol = 1e-12 if et != "P2" else 5e-12
+    for a, b, c in product(range(deg + 1), repeat=3):
+        if a + b + c <= deg:
+            val = np.sum(wf * xi[0] ** a * xi[1] ** b * xi[2] ** c)
+            assert abs(val - _mono_tet(a, b, c)) < tol, (a, b, c)
+
+
+def test_tri_quadrature_exact():
+    xi, wf = fem.quadrature_2d("P2")
+    for a, b in product(range(6), repeat=2):
+        if a + b <= 5:
+            exact = math.factorial(a) * math.factorial(b) / math.factorial(a + b + 2)
+            assert abs(np.sum(wf * xi[0] ** a * xi[1] ** b) - exact) < 1e-12
+
+
+@pytest.mark.parametrize("p", [1, 2, 4])
+def test_tet_basis_nodal_and_partition(p):
+    nodes = np.array([[t[1], t[2], t[3]] for t in fem.tetra_node_tuples(p)], dtype=float).T / p
+    if p == 4:
+        hatp, d1, d2, d3 = fem.tetra_lagrange_basis(4, nodes)
+    else:
+        hatp, d1, d2, d3 = fem.basis_3d(f"P{p}", nodes)
+    np.testing.assert_allclose(hatp, np.eye(nodes.shape[1]), atol=1e-12)
+    xi, _ = fem.quadrature_3d("P4")
+    h, g1, g2, g3 = fem.tetra_lagrange_basis(p, xi)
+    np.testing.assert_allclose(h.sum(0), 1.0, atol=1e-12)
+    for g in (g1, g2, g3):
+        np.testing.assert_allclose(g.sum(0), 0.0, atol=1e-11)
+
+
+def test_p2_closed_form_matches_general_lagrange():
+    xi, _ = fem.quadrature_3d("P2")
+    a = fem.basis_3d("P2", xi)
+    b = fem.tetra_lagrange_basis(2, xi)
+    for x, y in zip(a, b):
+        np.testing.assert_allclose(x, y, atol=1e-13)
+
+
+def test_reference_2d_slope_counts():
+    # benchmarks/run_2D_homo_SSR_capture run_info.json: 2721 nodes, 1300 elements, 5220 unknowns
+    m = mesh.slope_2d_p2(h=1.0, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0)
+    assert m.coord.shape == (2, 2721)
+    assert m.elem.shape == (6, 1300)
+    assert int(m.q_mask.sum()) == 5220
+
+
+def test_volumes_and_weights():
+    m = mesh.slope_2d_p2(h=1.0)
+    g = fem.geometry(m.coord, m.elem, "P2")
+    area = 40 * 10 + 0.5 * (15 + 25) * 10  # foundation + trapezoid
+    assert abs(g.weight.sum() - area) < 1e-9
+    m3 = mesh.slope_3d(h=2.5, nz=4)
+    g3 = fem.geometry(m3.coord, m3.elem, "P2")
+    assert abs(g3.weight.sum() - area * 40.0) < 1e-8
+    # gradients of a linear field are exact
+    u = 2.0 * m3.coord[0] - 3.0 * m3.coord[1] + 0.5 * m3.coord[2]
+    grad = np.stack([(g3.dphi[c] * u[m3.elem.T].repeat(g3.n_q, axis=0)).sum(1) for c in range(3)])
+    np.testing.assert_allclose(grad, np.array([[2.0], [-3.0], [0.5]]) * np.ones((1, g3.n_int)), atol=1e-10)
+
+
+def test_tet10_elevation_first_encounter_order():
+    coord = np.array([[0, 1, 0, 0, 1], [0, 0, 1, 0, 1], [0, 0, 0, 1, 1]], dtype=float)
+    elem = np.array([[0, 1], [1, 2], [2, 3], [3, 4]])
+    c10, e10 = mesh.elevate_tet4_to_tet10(coord, elem)
+    # element 0 edges (01,12,02,13,23,03) -> new nodes 5..10
+    np.testing.assert_array_equal(e10[4:, 0], [5, 6, 7, 8, 9, 10])
+    # element 1 = (1,2,3,4): edges 12 (seen: 6), 23 (seen: 9), 13 (seen: 8), 24 new, 34 new, 14 new
+    np.testing.assert_array_equal(e10[4:, 1], [6, 9, 8, 11, 12, 13])
+    np.testing.assert_allclose(c10[:, 11], 0.5 * (coord[:, 2] + coord[:, 4]))
+
+
+def test_xyz_order_is_x_major():
+    m = mesh.slope_3d(h=5.0, nz=2, x1=10, x2=10, x3=10, y1=10, y2=10, z_len=10)
+    c = m.coord
+    keys = np.lexsort((c[2], c[1], c[0]))
+    np.testing.assert_array_equal(keys, np.arange(c.shape[1]))

@@ -1,0 +1,284 @@
+"""Parity of the HIP kernels (called through the C-ABI) with the CPU oracle and the reference's
+golden vectors.  Tolerances: FP64 stress/tangent 1e-10 relative, assembly values
+1e-12 relative to the largest entry, integer structure bit-exact."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from slope_stability_petsc4py_amd import _lib, fem, mesh, pattern
+from slope_stability_petsc4py_amd import workloads as W
+from slope_stability_petsc4py_amd.constitutive import (ConstitutiveOperator, constitutive_problem_2d,
+                                                       constitutive_problem_3d, reduction)
+from slope_stability_petsc4py_amd.device import DevicePattern
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+RTOL = 1e-10
+
+
+def _rel(a, b):
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+def _rowrel(a, b):
+    """Max over points of |a-b|_inf / |b|_inf of that point."""
+    num = np.abs(a - b).max(axis=1)
+    den = np.maximum(np.abs(b).max(axis=1), 1e-300)
+    return num / den
+
+
+def test_constitutive_3d_golden(gpu):
+    print("case: constitutive_3d golden", flush=True)
+    d = np.load(GOLD / "constitutive_3d.npz")
+    S, DS = constitutive_problem_3d(d["E"], d["c_bar"], d["sin_phi"], d["shear"], d["lame"], return_tangent=True)
+    assert _rel(S.cpu().numpy(), d["S"]) < RTOL
+    assert _rel(DS.cpu().numpy(), d["DS"]) < RTOL
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 200_003])
+def test_constitutive_3d_stratified_vs_oracle(gpu, n):
+    print(f"case: constitutive_3d stratified n={n}", flush=True)
+    E, b = W.stratified_strains(n, seed=n)
+    mat = (np.full(n, 4 * np.sqrt(3)), np.full(n, 1 / np.sqrt(3)), np.full(n, W.HOMO.shear), np.full(n, W.HOMO.lame))
+    S, DS, DS32, br = constitutive_problem_3d(E, *mat, return_tangent=True, fp32_tangent=True, return_branch=True)
+    So, DSo, bro = oracle.constitutive_3d(E, *mat)
+    assert np.array_equal(br.cpu().numpy().astype(np.int32), bro)
+    assert _rel(S.cpu().numpy(), So) < RTOL
+    assert _rel(DS.cpu().numpy(), DSo) < RTOL
+    # per-point relative error, except near eigenvalue ties where the formula itself is ill-conditioned
+    err = _rowrel(DS.cpu().numpy(), DSo)
+    assert np.quantile(err, 0.999) < 1e-8
+    np.testing.assert_array_equal(DS32.cpu().numpy(), DS.cpu().numpy().astype(np.float32))
+
+
+def test_constitutive_3d_s_only_matches_sds(gpu):
+    E, _ = W.stratified_strains(5000, seed=2)
+    mat = (np.full(5000, 4 * np.sqrt(3)), np.full(5000, 1 / np.sqrt(3)), np.full(5000, W.HOMO.shear),
+           np.full(5000, W.HOMO.lame))
+    S1 = constitutive_problem_3d(E, *mat)
+    S2, _ = constitutive_problem_3d(E, *mat, return_tangent=True)
+    assert torch.equal(S1, S2)
+
+
+def test_constitutive_2d_vs_oracle(gpu):
+    print("case: constitutive_2d", flush=True)
+    rng = np.random.default_rng(9)
+    n = 3000
+    E = rng.normal(0, 1e-3, (n, 3))
+    E[:100] = 0.0                 # I2 == 0 tie branch
+    E[100:200, 2] = 0.0
+    E[100:200, 1] = E[100:200, 0]  # equal in-plane eigenvalues
+    cb, sp = 4 * np.sqrt(3), 1 / np.sqrt(3)
+    mat = (np.full(n, cb), np.full(n, sp), np.full(n, W.HOMO.shear), np.full(n, W.HOMO.bulk), np.full(n, W.HOMO.lame))
+    S, DS, br = constitutive_problem_2d(E, *mat, return_tangent=True, return_branch=True)
+    So, DSo, bro = oracle.mc2d(E, mat[0], mat[1], mat[2], mat[4])
+    assert np.array_equal(br.cpu().numpy().astype(np.int32), bro)
+    assert len(set(bro.tolist())) >= 4
+    assert _rel(S.cpu().numpy(), So) < RTOL
+    assert _rel(DS.cpu().numpy(), DSo) < RTOL
+
+
+def test_reduction_vs_numpy(gpu):
+    rng = np.random.default_rng(1)
+    n = 1000
+    c0 = rng.uniform(5, 20, n)
+    phi = np.deg2rad(rng.uniform(20, 45, n))
+    psi = np.deg2rad(rng.uniform(0, 10, n))
+    for typ in "ABC":
+        for lam in (0.7, 1.0, 1.66):
+            cb, sp = reduction(c0, phi, psi, lam, typ)
+            cbo, spo = oracle.davis_reduction(c0, phi, psi, lam, typ)
+            assert _rel(cb.cpu().numpy(), cbo) < 1e-13
+            assert _rel(sp.cpu().numpy(), spo) < 1e-13
+    with pytest.raises(ValueError):
+        reduction(c0, phi, psi, 0.0, "B")
+    with pytest.raises(ValueError):
+        reduction(c0, phi, psi, 1.0, "D")
+
+
+@pytest.fixture(scope="module")
+def asm3d():
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=4)
+    geom = fem.geometry(m.coord, m.elem, m.elem_type)
+    pat = pattern.build_owned_pattern(m.elem, m.q_mask)
+    return m, geom, pat
+
+
+def test_assembly_3d_golden(gpu):
+    print("case: assembly golden", flush=True)
+    d = np.load(GOLD / "assembly_3d.npz")
+    m = mesh.slope_3d(h=5.0, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=2)
+    geom = fem.geometry(m.coord, m.elem, m.elem_type)
+    pat = pattern.build_owned_pattern(m.elem, m.q_mask)
+    geom.dphi, geom.weight = d["dphi"], d["weight"]
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    E = torch.empty((geom.n_int, 6), dtype=torch.float64, device=dev)
+    dp.strain(torch.from_numpy(d["u"]).to(dev), E)
+    assert _rel(E.cpu().numpy(), d["E"]) < 1e-12
+    fe = torch.empty(geom.n_elem * 30, dtype=torch.float64, device=dev)
+    F = torch.empty(pat.n_rows, dtype=torch.float64, device=dev)
+    dp.force(torch.from_numpy(d["S"]).to(dev), fe, F)
+    assert _rel(F.cpu().numpy(), d["F"]) < 1e-12
+    K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    dp.tangent(torch.from_numpy(d["DS"]).to(dev), K)
+    assert _rel(K.cpu().numpy(), d["K"]) < 1e-12
+
+
+def test_assembly_3d_vs_oracle_and_deterministic(gpu, asm3d):
+    print("case: assembly 3D vs oracle", flush=True)
+    m, geom, pat = asm3d
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    rng = np.random.default_rng(21)
+    u = rng.normal(0, 1e-3, 3 * m.n_nodes)
+    E = torch.empty((geom.n_int, 6), dtype=torch.float64, device=dev)
+    dp.strain(torch.from_numpy(u).to(dev), E)
+    Eo = oracle.strain_3d(geom.dphi, u, pat.elem_dofs, geom.n_q)
+    assert _rel(E.cpu().numpy(), Eo) < 1e-12
+    A = rng.normal(size=(geom.n_int, 6, 6))
+    DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
+    S = rng.normal(size=(geom.n_int, 6))
+    fe = torch.empty(geom.n_elem * 30, dtype=torch.float64, device=dev)
+    F = torch.empty(pat.n_rows, dtype=torch.float64, device=dev)
+    dp.force(torch.from_numpy(S).to(dev), fe, F)
+    assert _rel(F.cpu().numpy(), oracle.force_3d_rows(geom.dphi, S, geom.weight, pat, geom.n_q)) < 1e-12
+    K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    K32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev)
+    DSd = torch.from_numpy(DS).to(dev)
+    dp.tangent(DSd, K, K32)
+    Ko = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+    assert _rel(K.cpu().numpy(), Ko) < 1e-12
+    np.testing.assert_array_equal(K32.cpu().numpy(), K.cpu().numpy().astype(np.float32))
+    # constrained diagonal = 1
+    assert np.all(K.cpu().numpy()[pat.constrained_diag_positions] == 1.0)
+    # run twice: bitwise identical (no atomics, fixed summation order)
+    K2 = torch.empty_like(K)
+    dp.tangent(DSd, K2)
+    assert torch.equal(K, K2)
+    # regularised operator K_r = (1-r) K + r K_el
+    el = torch.from_numpy(rng.normal(size=pat.nnz)).to(dev)
+    el[torch.from_numpy(pat.constrained_diag_positions.astype(np.int64)).to(dev)] = 1.0
+    Kr = torch.empty_like(K)
+    dp.tangent(DSd, Kr, r=0.3, elastic=el)
+    np.testing.assert_allclose(Kr.cpu().numpy(), 0.7 * Ko + 0.3 * el.cpu().numpy(), rtol=1e-12,
+                               atol=1e-12 * np.abs(Ko).max())
+
+
+def test_tangent_global_fallback_path(gpu, asm3d):
+    """The LDS-tiled and the global-memory variants of the tangent kernel agree bitwise."""
+    m, geom, pat = asm3d
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    DS = torch.from_numpy(np.random.default_rng(2).normal(size=(geom.n_int, 36))).to(dev)
+    K1 = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    K2 = torch.empty_like(K1)
+    dp.tangent(DS, K1)
+    dp.max_block_nnz = -1  # forces the global-memory path
+    dp.tangent(DS, K2)
+    assert torch.equal(K1, K2)
+
+
+def test_spmv_vs_oracle(gpu, asm3d):
+    m, geom, pat = asm3d
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    rng = np.random.default_rng(5)
+    val = rng.normal(size=pat.nnz)
+    x = rng.normal(size=pat.n_rows)
+    K = dp.new_csr()
+    K.values.copy_(torch.from_numpy(val))
+    K.values32.copy_(torch.from_numpy(val.astype(np.float32)))
+    y = K.matvec(torch.from_numpy(x).to(dev))
+    yo = oracle.spmv(pat.indptr, pat.indices, val, x)
+    assert _rel(y.cpu().numpy(), yo) < 1e-13
+    y32 = K.matvec32(torch.from_numpy(x.astype(np.float32)).to(dev))
+    assert _rel(y32.cpu().numpy().astype(np.float64), yo) < 1e-5
+    for lanes in (1, 2, 4, 8, 16, 32, 64):
+        K.lanes = lanes
+        assert _rel(K.matvec(torch.from_numpy(x).to(dev)).cpu().numpy(), yo) < 1e-13
+
+
+def test_vector_kernels(gpu):
+    dev = torch.device("cuda", 0)
+    rng = np.random.default_rng(8)
+    n, k = 100_003, 7
+    V = torch.from_numpy(rng.normal(size=(k, n))).to(dev)
+    x = torch.from_numpy(rng.normal(size=n)).to(dev)
+    out = torch.empty(k, dtype=torch.float64, device=dev)
+    _lib.call("ssx_dot_cols", n, k, V.data_ptr(), n, x.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    np.testing.assert_allclose(out.cpu().numpy(), V.cpu().numpy() @ x.cpu().numpy(), rtol=1e-12)
+    part = torch.empty(1024, dtype=torch.float64, device=dev)
+    d = torch.empty(1, dtype=torch.float64, device=dev)
+    _lib.call("ssx_dot", n, x.data_ptr(), x.data_ptr(), part.data_ptr(), d.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    np.testing.assert_allclose(d.item(), float(x.cpu().numpy() @ x.cpu().numpy()), rtol=1e-12)
+    h = torch.from_numpy(rng.normal(size=k)).to(dev)
+    y = x.clone()
+    _lib.call("ssx_gemv_cols", n, k, V.data_ptr(), n, h.data_ptr(), -1.0, y.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    np.testing.assert_allclose(y.cpu().numpy(), x.cpu().numpy() - h.cpu().numpy() @ V.cpu().numpy(), rtol=1e-12,
+                               atol=1e-12)
+
+
+def test_operator_api_3d(gpu):
+    """ConstitutiveOperator methods on a small 3D slope vs the oracle pipeline."""
+    print("case: operator API 3D", flush=True)
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=4)
+    op = ConstitutiveOperator.from_mesh(m, [W.HOMO], Davis_type="B")
+    geom, pat = op.geom, op.pattern.host
+    rng = np.random.default_rng(3)
+    U = rng.normal(0, 2e-3, (3, m.n_nodes)) * m.q_mask
+    op.reduction(1.2)
+    F, K = op.build_F_K_tangent_reduced(U)
+    c0 = np.full(geom.n_int, 6.0)
+    cb, sp = oracle.davis_reduction(c0, np.full(geom.n_int, np.pi / 4), np.zeros(geom.n_int), 1.2, "B")
+    E = oracle.strain_3d(geom.dphi, U.reshape(-1, order="F"), pat.elem_dofs, geom.n_q)
+    S, DS, br = oracle.constitutive_3d(E, cb, sp, np.full(geom.n_int, W.HOMO.shear), np.full(geom.n_int, W.HOMO.lame))
+    assert len(np.unique(br)) >= 3
+    Fo = oracle.force_3d_rows(geom.dphi, S, geom.weight, pat, geom.n_q)
+    assert isinstance(F, np.ndarray) and F.shape == (3, m.n_nodes)
+    assert _rel(F.reshape(-1, order="F"), Fo) < 1e-10
+    Ko = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+    assert _rel(K.values.cpu().numpy(), Ko) < 1e-10
+    Ff = op.build_F_free()
+    np.testing.assert_array_equal(Ff, F.reshape(-1, order="F")[pat.free_idx])
+    Kr = op.build_K_regularized(0.25)
+    Kel = oracle.tangent_3d_rows(geom.dphi, np.tile(
+        (np.diag([2 * W.HOMO.shear] * 3 + [W.HOMO.shear] * 3) + np.pad(np.full((3, 3), W.HOMO.lame), ((0, 3), (0, 3))))
+        .reshape(-1, order="F"), (geom.n_int, 1)), geom.weight, pat, geom.n_q)
+    assert _rel(op.elastic_values.cpu().numpy(), Kel) < 1e-12
+    assert _rel(Kr.values.cpu().numpy(), 0.75 * Ko + 0.25 * Kel) < 1e-10
+    t = op.get_total_time()
+    assert set(t) >= {"reduction", "stress_tangent", "build_F", "build_tangent_local"}
+    with pytest.raises(ValueError):
+        ConstitutiveOperator.from_mesh(m, [W.HOMO]).constitutive_problem_stress(U)
+
+
+def test_operator_api_2d(gpu):
+    print("case: operator API 2D", flush=True)
+    m = mesh.slope_2d_p2(h=2.0, x1=6.0, x2=4.0, x3=6.0, y1=4.0, y2=4.0)
+    op = ConstitutiveOperator.from_mesh(m, [W.HOMO], Davis_type="B")
+    geom, pat = op.geom, op.pattern.host
+    B = oracle.strain_operator(geom.dphi, m.elem, m.n_nodes, geom.n_q)
+    rng = np.random.default_rng(4)
+    U = rng.normal(0, 2e-3, (2, m.n_nodes)) * m.q_mask
+    op.reduction(1.1)
+    F, K = op.build_F_K_tangent_reduced(U)
+    u = U.reshape(-1, order="F")
+    E = (B @ u).reshape(-1, 3)
+    cb, sp = oracle.davis_reduction(np.full(geom.n_int, 6.0), np.full(geom.n_int, np.pi / 4), np.zeros(geom.n_int),
+                                    1.1, "B")
+    S, DS, br = oracle.mc2d(E, cb, sp, np.full(geom.n_int, W.HOMO.shear), np.full(geom.n_int, W.HOMO.lame))
+    Fo = B.T @ (geom.weight[:, None] * S[:, :3]).reshape(-1)
+    Fo[pat.row_free == 0] = 0
+    assert _rel(F.reshape(-1, order="F"), Fo) < 1e-10
+    import scipy.sparse as sp_
+    blocks = [sp_.csr_matrix(geom.weight[g] * DS[g].reshape(3, 3, order="F")) for g in range(geom.n_int)]
+    Kg = (B.T @ sp_.block_diag(blocks, format="csr") @ B).tocsr()
+    rows = np.repeat(np.arange(pat.n_rows), np.diff(pat.indptr))
+    Ko = np.asarray(Kg[rows, pat.indices]).ravel()
+    Ko[pat.constrained_diag_positions] = 1.0
+    assert _rel(K.values.cpu().numpy(), Ko) < 1e-10
