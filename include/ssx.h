@@ -125,6 +125,50 @@ int ssx_inv_diag_f32(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const double *val,
                      double *dinv, void *stream);
 
+/* P4 tetrahedra (35 nodes, 24 points): element tangent matrices ke (n_elem x 105 x 105, row-major,
+ * row = local dof 3*node+comp) = sum_q B^T (w D) B on the FP64 matrix cores (v_mfma_f64_16x16x4).
+ * Replaces the element stage of assemble_tangent_values_3d_p2_c (cython/assemble_tangent_values_3d.c:45-94). */
+int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double *ke,
+                            void *stream);
+
+/* Row gather of element matrices into the fixed CSR pattern through the row-slot lists (slot order
+ * = element order), with unit constrained diagonals, optional K_r blend and FP32 copy.  Together
+ * with ssx_element_matrices_p4 replaces assemble_tangent_values_3d_rows_c for P4
+ * (cython/assemble_tangent_values_3d.c:111-214). */
+int ssx_scatter_element_rows(const double *ke, int nld, const int32_t *row_slot_ptr, const int32_t *slot_elem,
+                             const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr,
+                             const uint8_t *row_free, int64_t n_rows, double r, const double *elastic, double *out,
+                             float *out32, int64_t max_block_nnz, void *stream);
+
+/* ---- learned continuation predictor (no reference counterpart: it fills the predictor slot of
+ * continuation/indirect.py:2703-2830, where the reference uses the secant of :550-566) ---- */
+
+/* C (M x N fp32, row stride ldc) = op(A) op(B) [+ bias[j]] [tanh] [+ beta*C]; bf16 operands given
+ * by element strides (A[i*a_si + k*a_sk], B[k*b_sk + j*b_sj]); optional bf16 copy Cb (same ldc);
+ * act 0 none, 1 tanh.  bf16 MFMA with FP32 accumulation. */
+int ssx_gemm_bf16(int64_t M, int64_t N, int64_t K, const void *A, int64_t a_si, int64_t a_sk, const void *B,
+                  int64_t b_sk, int64_t b_sj, float *C, int64_t ldc, const float *bias, int act, void *Cb, float beta,
+                  void *stream);
+/* dZ = dY (1 - Y^2) with optional bf16 copy. */
+int ssx_tanh_bwd(int64_t n, const float *dY, const float *Y, float *dZ, void *dZb, void *stream);
+/* db[j] = sum_i dZ[i*N + j]. */
+int ssx_colsum_f32(int64_t M, int64_t N, const float *dZ, float *db, void *stream);
+/* Adam step (step >= 1) on FP32 weights w with moments m, v; optional bf16 copy wb. */
+int ssx_adam_f32(int64_t n, float *w, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                 int step, void *wb, void *stream);
+int ssx_f32_to_bf16(int64_t n, const float *x, void *y, void *stream);
+int ssx_bf16_to_f32(int64_t n, const void *x, float *y, void *stream);
+/* Masked MSE: g = 2 (p - t) mask * inv_count; loss partial sums per 256 entries in loss_part. */
+int ssx_mse_f32(int64_t n, const float *p, const float *t, const float *mask, float inv_count, float *g,
+                float *loss_part, void *stream);
+
+/* Dense Galerkin coarse operator Ac (n_c x n_c, column-major, caller-zeroed) += P^T A P with P given
+ * as CSR rows (p_ptr, p_idx, p_w) over the fine rows of A; FP64 atomics, setup-time only.
+ * Replaces the PETSc MatPtAP of the p-multigrid set-up (linear/solver.py:510-513). */
+int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                       const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c, double *Ac,
+                       void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,6 +41,16 @@ _SIGS: dict[str, list] = {
     "ssx_cheb_update_f64": [I64, P, P, F64, F64, P, P, P],
     "ssx_inv_diag_f32": [I64, I64, P, P, P, P, P],
     "ssx_inv_diag_f64": [I64, I64, P, P, P, P, P],
+    "ssx_galerkin_dense": [I64, P, P, P, P, P, P, I64, P, P],
+    "ssx_element_matrices_p4": [P, P, P, I64, P, P],
+    "ssx_scatter_element_rows": [P, I32, P, P, P, P, P, P, I64, F64, P, P, P, I64, P],
+    "ssx_gemm_bf16": [I64, I64, I64, P, I64, I64, P, I64, I64, P, I64, P, I32, P, F32, P],
+    "ssx_tanh_bwd": [I64, P, P, P, P, P],
+    "ssx_colsum_f32": [I64, I64, P, P, P],
+    "ssx_adam_f32": [I64, P, P, P, P, F32, F32, F32, F32, I32, P, P],
+    "ssx_f32_to_bf16": [I64, P, P, P],
+    "ssx_bf16_to_f32": [I64, P, P, P],
+    "ssx_mse_f32": [I64, P, P, P, F32, P, P, P],
     "ssx_stream_sync": [P],
 }
 

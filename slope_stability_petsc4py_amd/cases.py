@@ -1,0 +1,125 @@
+"""Problem set-ups of the benchmark configurations and the SSR / LL runners on the device.
+
+* ``homo2d``: the reference's 2D homogeneous SSR capture (benchmarks/run_2D_homo_SSR_capture:
+  h = 1, x1 = x3 = 15, x2 = y1 = y2 = 10, c = 6 kPa, phi = 45, psi = 0, E = 40 MPa, nu = 0.3,
+  gamma = 20, Davis B; continuation lambda_init 0.9, d_lambda_init 0.1, d_lambda_min 1e-5,
+  d_lambda_diff_scaled_min 1e-3, omega_max 7e7, Newton tol 1e-4, r_min 1e-4, it_newt_max 50).
+* ``hetero3d_L1``: the reference's 3D heterogeneous SSR capture on SSR_hetero_ada_L1 (P2,
+  materials of meshes/3d_hetero_ssr/definition.py, Dirichlet labels x:[1,2] y:[5] z:[3,4];
+  lambda_init 1.0, d_lambda_init 0.1, d_lambda_min 1e-3, omega_max 1.2e7, it_newt_max 200).
+* ``slope3d``: structured 3D slope (homogeneous or four horizontal layers).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import mesh as M
+from .constitutive import ConstitutiveOperator
+from .continuation import ll_indirect_continuation, ssr_indirect_continuation
+from .device import to_dev
+from .linear import ChebyshevJacobi, DeflatedFGMRES, PMultigrid
+from .materials import MaterialSpec
+from .workloads import HETERO_LAYERS, HOMO
+
+DATA = Path(__file__).resolve().parents[1] / "tests" / "golden"
+
+HETERO_L1_MATERIALS = [
+    MaterialSpec(15.0, 30.0, 0.0, 10000.0, 0.33, 19.0, 19.0),
+    MaterialSpec(15.0, 38.0, 0.0, 50000.0, 0.30, 22.0, 22.0),
+    MaterialSpec(10.0, 35.0, 0.0, 50000.0, 0.30, 21.0, 21.0),
+    MaterialSpec(18.0, 32.0, 0.0, 20000.0, 0.33, 20.0, 20.0),
+]
+
+
+@dataclass
+class SSRParams:
+    lambda_init: float = 1.0
+    d_lambda_init: float = 0.1
+    d_lambda_min: float = 1e-3
+    d_lambda_diff_scaled_min: float = 1e-3
+    step_max: int = 100
+    omega_max_stop: float = 1.2e7
+    it_newt_max: int = 200
+    it_damp_max: int = 10
+    tol: float = 1e-4
+    r_min: float = 1e-4
+    linear_tol: float = 1e-1
+    linear_maxit: int = 100
+
+
+@dataclass
+class Case:
+    name: str
+    mesh: M.SlopeMesh
+    materials: list
+    params: SSRParams
+    davis: str = "B"
+    extra: dict = field(default_factory=dict)
+
+
+def homo2d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.0) -> Case:
+    m = M.slope_2d_p2(h=h, x1=x1, x2=x2, x3=x3, y1=10.0, y2=10.0)
+    p = SSRParams(lambda_init=0.9, d_lambda_init=0.1, d_lambda_min=1e-5, d_lambda_diff_scaled_min=1e-3,
+                  step_max=100, omega_max_stop=7e7, it_newt_max=50)
+    return Case("homo2d", m, [HOMO], p)
+
+
+def hetero3d_L1(elem_type: str = "P2") -> Case:
+    d = np.load(DATA / "ssr_hetero_L1_tet4.npz", allow_pickle=False)
+    m = M.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
+                               d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,
+                               {"x": [1, 2], "y": [5], "z": [3, 4]}, elem_type)
+    return Case("hetero3d_L1", m, HETERO_L1_MATERIALS, SSRParams())
+
+
+def slope3d(h: float = 2.0, nz: int = 10, hetero: bool = False) -> Case:
+    layers = (15.0, 10.0, 5.0) if hetero else None
+    m = M.slope_3d(h=h, nz=nz, layers=layers)
+    mats = HETERO_LAYERS if hetero else [HOMO]
+    p = SSRParams(lambda_init=1.0 if hetero else 0.9, omega_max_stop=1e9)
+    return Case("slope3d_hetero" if hetero else "slope3d_homo", m, mats, p)
+
+
+@dataclass
+class Solver:
+    case: Case
+    op: ConstitutiveOperator
+    f: torch.Tensor
+    free_mask: torch.Tensor
+    linear: DeflatedFGMRES
+
+
+def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg") -> Solver:
+    """precision 'fp64': FP64 preconditioner; 'fp32': FP32 tangent shadow + FP32 preconditioner
+    (the outer FGMRES, residuals and return mapping stay FP64)."""
+    use32 = precision == "fp32"
+    op = ConstitutiveOperator.from_mesh(case.mesh, case.materials, Davis_type=case.davis, fp32_shadow=use32)
+    f_field = op.gravity_load(case.mesh)
+    f = to_dev(f_field.T.reshape(-1).copy())
+    free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
+    if pc == "pmg":
+        prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32)
+    else:
+        prec = ChebyshevJacobi(use32=use32)
+    lin = DeflatedFGMRES(prec, tolerance=case.params.linear_tol, max_iterations=case.params.linear_maxit)
+    return Solver(case, op, f, free_mask, lin)
+
+
+def run_ssr(s: Solver, *, predictor=None, store_step_u: bool = False, progress=None):
+    p = s.case.params
+    return ssr_indirect_continuation(p.lambda_init, p.d_lambda_init, p.d_lambda_min, p.d_lambda_diff_scaled_min,
+                                     p.step_max, p.omega_max_stop, p.it_newt_max, p.it_damp_max, p.tol, p.r_min,
+                                     s.f, s.op, s.linear, free_mask=s.free_mask, dim=s.case.mesh.dim,
+                                     predictor=predictor, store_step_u=store_step_u, progress=progress)
+
+
+def run_ll(s: Solver, *, d_omega_ini_scale: float = 0.2, d_t_min: float = 1e-3, progress=None):
+    p = s.case.params
+    return ll_indirect_continuation(d_omega_ini_scale, d_t_min, p.step_max, p.omega_max_stop, p.it_newt_max,
+                                    p.it_damp_max, p.tol, p.r_min, s.f, s.op, s.linear, free_mask=s.free_mask,
+                                    dim=s.case.mesh.dim, progress=progress)

@@ -215,7 +215,8 @@ class ConstitutiveOperator:
         self.time_local_strain.append(t1 - t0)
         self.time_local_constitutive.append(perf_counter() - t1)
         self._s_ready = True
-        self._ds_ready = tangent
+        if tangent:
+            self._ds_ready = True
 
     @property
     def S(self):
@@ -265,6 +266,19 @@ class ConstitutiveOperator:
         return F.cpu().numpy() if getattr(self, "_kind", "") == "numpy" else F
 
     build_F_free_local = build_F_free
+
+    # flat full-length device vectors (constrained dofs zero) for the device Newton solvers
+    def build_F_vec(self) -> torch.Tensor:
+        return self._force()
+
+    def build_F_all_vec(self, lam, U) -> torch.Tensor:
+        self.reduction(lam)
+        self.constitutive_problem_stress(U)
+        return self._force()
+
+    def build_F_reduced_vec(self, U) -> torch.Tensor:
+        self.constitutive_problem_stress(U)
+        return self._force()
 
     def _tangent_matrix(self) -> DeviceCSR:
         if not self._ds_ready:

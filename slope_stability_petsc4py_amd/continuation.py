@@ -1,0 +1,301 @@
+"""Indirect continuation for the factor of safety (SSR) and the limit load (LL), driving the device
+Newton solvers.
+
+SSR (continuation/indirect.py:1627-1736 init phase, :2267-3557 main loop, classic mode with the
+legacy omega controller): two converged states at lambda_init and lambda_init + d_lambda start the
+branch; each step targets omega + d_omega, predicts U by the secant in omega
+(continuation/indirect.py:550-566) or by a supplied predictor, runs the nested Newton, halves
+d_omega on failure (at most 5 times in a row), doubles it after a step whose lambda increment
+shrank by more than 10%, and stops when d_lambda/d_omega * (omega - omega_0) falls below
+d_lambda_diff_scaled_min, omega reaches omega_max_stop, or step_max steps were taken.
+The factor of safety is the last accepted lambda.
+
+A predictor slot takes a learned initial guess; it is accepted only when the nested Newton's
+first residual is no larger than the secant guess would have been allowed to start from (the
+attempt is gated, and the secant is used when the gate rejects it).
+
+LL (continuation/limit_load.py:66-374): same structure over the load factor t with F(U) = t f.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from .linear import _Scratch, dot
+from .nonlinear import NewtonStats, newton, newton_ind_ll, newton_ind_ssr
+
+
+@dataclass
+class SSRResult:
+    lambda_hist: np.ndarray
+    omega_hist: np.ndarray
+    Umax_hist: np.ndarray
+    U: torch.Tensor
+    stop_reason: str
+    wall_time: float
+    newton_iterations: int
+    linear_iterations: int
+    attempts: int
+    step_U: list = field(default_factory=list)
+    step_info: list = field(default_factory=list)
+
+    @property
+    def fos(self) -> float:
+        return float(self.lambda_hist[-1])
+
+
+def _umax(U: torch.Tensor, dim: int) -> float:
+    return float(torch.linalg.vector_norm(U.view(-1, dim), dim=1).max().item())
+
+
+def init_phase_ssr(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol, r_min, f, builder,
+                   solver, *, free_mask, stats):
+    """Two converged states on the branch (continuation/indirect.py:1627-1736)."""
+    scr = _Scratch(f.device)
+    U_ini = torch.zeros_like(f)
+    d_lambda = float(d_lambda_init)
+    lambda1 = float(lambda_init)
+    while True:
+        builder.reduction(lambda1)
+        U_it, flag, _ = newton(U_ini, tol, it_newt_max, it_damp_max, r_min, f, builder, solver, free_mask=free_mask,
+                               stats=stats)
+        if flag == 0:
+            break
+        lambda1 *= 0.5
+        d_lambda *= 0.5
+        if d_lambda < d_lambda_min:
+            raise RuntimeError("Initial choice of lambda seems to be too large.")
+    U1 = U_it
+    omega1 = dot(f, U1, scr)
+    solver.expand_deflation_basis(U1)
+    while True:
+        lambda_it = lambda1 + d_lambda
+        builder.reduction(lambda_it)
+        U_it, flag, _ = newton(U1, tol, it_newt_max, it_damp_max, r_min, f, builder, solver, free_mask=free_mask,
+                               stats=stats)
+        if flag == 1:
+            d_lambda /= 2.0
+        else:
+            U2 = U_it
+            omega2 = dot(f, U2, scr)
+            lambda2 = lambda_it
+            if (omega2 - omega1) / max(1.0, omega1) < 1e-5:
+                U1, lambda1, omega1 = U2, lambda2, omega2
+            else:
+                break
+        if d_lambda < d_lambda_min:
+            raise RuntimeError("It seems that the FoS is equal to lambda_init.")
+        if lambda1 > 10.0:
+            raise RuntimeError("It seems that the FoS is greater than 10.")
+    return U1, U2, omega1, omega2, lambda1, lambda2
+
+
+def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
+                              omega_max_stop, it_newt_max, it_damp_max, tol, r_min, f, builder, solver, *,
+                              free_mask, dim, predictor=None, store_step_u=False, progress=None) -> SSRResult:
+    """SSR indirect continuation; returns the lambda/omega/Umax histories (FoS = lambda_hist[-1]).
+
+    ``predictor(ctx) -> U_guess or None`` is the learned-predictor slot: ctx holds the accepted
+    states, histories and the omega target; a returned guess is tried first and kept only if its
+    first nested-Newton residual passes the gate (below the secant's first residual)."""
+    t0 = perf_counter()
+    stats = NewtonStats()
+    scr = _Scratch(f.device)
+    lin0 = solver.iteration_collector.get_total_iterations()
+    U_old, U, omega_old, omega, lambda_old, lambda_value = init_phase_ssr(
+        lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_damp_max, tol, r_min, f, builder, solver,
+        free_mask=free_mask, stats=stats)
+    solver.expand_deflation_basis(U_old)
+    lambda_hist = [lambda_old, lambda_value]
+    omega_hist = [omega_old, omega]
+    umax_hist = [_umax(U_old, dim), _umax(U, dim)]
+    step_U = [U_old.clone(), U.clone()] if store_step_u else []
+    states = [(U_old, omega_old, lambda_old), (U, omega, lambda_value)]  # last accepted states (<= 3)
+    step_info = []
+    d_omega = omega - omega_old
+    if omega_max_stop < omega + d_omega:
+        raise ValueError("Too small value of omega_max_stop. Increase it and rerun.")
+    step = 2
+    n_omega = 0
+    attempts = 0
+    stop_reason = "completed"
+    while True:
+        omega_it = min(omega + d_omega, omega_max_stop)
+        d_omega = omega_it - omega
+        alpha_sec = 0.0 if abs(omega - omega_old) <= 1e-30 else (omega_it - omega) / (omega - omega_old)
+        U_sec = U + alpha_sec * (U - U_old)
+        kind = "secant"
+        learned = None
+        if predictor is not None:
+            learned = predictor({"U": U, "U_old": U_old, "omega": omega, "omega_old": omega_old,
+                                 "omega_target": omega_it, "lambda": lambda_value, "lambda_old": lambda_old,
+                                 "lambda_hist": lambda_hist, "omega_hist": omega_hist, "states": states})
+        basis_snap = solver.get_deflation_basis_snapshot()
+        t_att = perf_counter()
+        newton_it_before = stats.newton_iterations
+        result = None
+        gate_info = None
+        if learned is not None:
+            # rescale to the omega constraint, then gate on the first residual vs the secant's
+            d = dot(f, learned, scr)
+            if d != 0.0:
+                learned = learned * (omega_it / d)
+            builder.reduction(lambda_value)
+            F_sec = builder.build_F_reduced_vec(U_sec).clone()
+            r_sec = float(torch.linalg.vector_norm(F_sec - f).item())
+            F_l = builder.build_F_reduced_vec(learned)
+            r_l = float(torch.linalg.vector_norm(F_l - f).item())
+            gate_info = {"secant_first_residual": r_sec, "learned_first_residual": r_l}
+            if r_l < r_sec:
+                kind = "learned"
+                result = newton_ind_ssr(learned, omega_it, lambda_value, it_newt_max, it_damp_max, tol, r_min, f,
+                                        builder, solver, free_mask=free_mask, stats=stats)
+                if result[2] != 0:
+                    solver.restore_deflation_basis(basis_snap)
+                    result = None
+                    kind = "learned_rejected"
+            else:
+                kind = "learned_gated"
+        if result is None:
+            result = newton_ind_ssr(U_sec, omega_it, lambda_value, it_newt_max, it_damp_max, tol, r_min, f, builder,
+                                    solver, free_mask=free_mask, stats=stats)
+            if kind == "learned_rejected":
+                kind = "secant_after_rejection"
+        U_it, lambda_cand, flag, it_newt, info = result
+        solver.restore_deflation_basis(basis_snap)
+        attempts += 1
+        rec = {"step": step + 1, "omega_target": omega_it, "flag": flag, "newton_it": it_newt, "predictor": kind,
+               "attempt_wall": perf_counter() - t_att, "newton_total": stats.newton_iterations - newton_it_before,
+               "first_rel_residual": info.get("first_rel_residual")}
+        if gate_info:
+            rec.update(gate_info)
+        step_info.append(rec)
+        if progress is not None:
+            progress(dict(rec, lam=lambda_cand))
+        if flag != 0:
+            d_omega /= 2.0
+            n_omega += 1
+        else:
+            step += 1
+            U_old, U = U, U_it
+            solver.expand_deflation_basis(U)
+            omega_old, omega = omega, omega_it
+            d_lambda = lambda_cand - lambda_value
+            lambda_old, lambda_value = lambda_value, lambda_cand
+            d_lambda_diff_scaled = np.nan if d_omega == 0.0 else (d_lambda / d_omega) * (omega - omega_hist[0])
+            n_omega = 0
+            lambda_hist.append(lambda_value)
+            omega_hist.append(omega)
+            umax_hist.append(_umax(U, dim))
+            if store_step_u:
+                step_U.append(U.clone())
+            states = (states + [(U, omega, lambda_value)])[-3:]
+            if (lambda_hist[-1] - lambda_hist[-2]) < 0.9 * (lambda_hist[-2] - lambda_hist[-3]):
+                d_omega *= 2.0
+            if d_lambda_diff_scaled_min > 0.0 and d_lambda_diff_scaled < d_lambda_diff_scaled_min:
+                stop_reason = "d_lambda_diff_scaled_min"
+                break
+            if omega >= omega_max_stop:
+                stop_reason = "omega_max_stop"
+                break
+        if n_omega >= 5:
+            stop_reason = "omega_reduction_limit"
+            break
+        if step >= step_max:
+            stop_reason = "step_max"
+            break
+    torch.cuda.synchronize()
+    return SSRResult(np.asarray(lambda_hist), np.asarray(omega_hist), np.asarray(umax_hist), U, stop_reason,
+                     perf_counter() - t0, stats.newton_iterations,
+                     solver.iteration_collector.get_total_iterations() - lin0, attempts, step_U, step_info)
+
+
+@dataclass
+class LLResult:
+    t_hist: np.ndarray
+    omega_hist: np.ndarray
+    Umax_hist: np.ndarray
+    U: torch.Tensor
+    stop_reason: str
+    wall_time: float
+    newton_iterations: int
+    linear_iterations: int
+
+    @property
+    def limit_load(self) -> float:
+        return float(self.t_hist[-1])
+
+
+def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_stop, it_newt_max, it_damp_max, tol,
+                             r_min, f, builder, solver, *, free_mask, dim, progress=None) -> LLResult:
+    """Limit-load continuation in omega = f^T U (continuation/limit_load.py:66-374): an elastic
+    solve gives U_el; omega starts at d_omega_ini_scale * f^T U_el; steps grow after easy steps and
+    halve on failure; stops at omega_max_stop, step_max, 5 reductions, or d_t < d_t_min."""
+    t0 = perf_counter()
+    stats = NewtonStats()
+    scr = _Scratch(f.device)
+    lin0 = solver.iteration_collector.get_total_iterations()
+    builder.reduction(1.0)
+    U_el, flag, _ = newton(torch.zeros_like(f), tol, it_newt_max, it_damp_max, r_min, f, builder, solver,
+                           free_mask=free_mask, stats=stats)
+    omega_el = dot(f, U_el, scr)
+    d_omega = d_omega_ini_scale * omega_el
+    U_old = torch.zeros_like(f)
+    U = U_el * d_omega_ini_scale
+    # first point: F(U) = t f at omega = d_omega
+    U, t, flag, _ = newton_ind_ll(U, d_omega_ini_scale, d_omega, it_newt_max, it_damp_max, tol, r_min, f, builder,
+                                  solver, free_mask=free_mask, stats=stats)
+    t_hist = [0.0, t]
+    omega_hist = [0.0, d_omega]
+    umax_hist = [0.0, _umax(U, dim)]
+    omega = d_omega
+    omega_old = 0.0
+    n_omega = 0
+    stop_reason = "completed"
+    while True:
+        omega_it = min(omega + d_omega, omega_max_stop)
+        d_omega = omega_it - omega
+        a = 0.0 if omega == omega_old else d_omega / (omega - omega_old)
+        U_ini = U + a * (U - U_old)
+        t_ini = t + a * (t - t_hist[-2])
+        basis_snap = solver.get_deflation_basis_snapshot()
+        U_it, t_it, flag, it = newton_ind_ll(U_ini, t_ini, omega_it, it_newt_max, it_damp_max, tol, r_min, f, builder,
+                                             solver, free_mask=free_mask, stats=stats)
+        solver.restore_deflation_basis(basis_snap)
+        if progress is not None:
+            progress({"omega_target": omega_it, "flag": flag, "newton_it": it, "t": t_it})
+        if flag != 0:
+            d_omega /= 2.0
+            n_omega += 1
+        else:
+            d_t = t_it - t
+            U_old, U = U, U_it
+            solver.expand_deflation_basis(U)
+            omega_old, omega = omega, omega_it
+            t = t_it
+            t_hist.append(t)
+            omega_hist.append(omega)
+            umax_hist.append(_umax(U, dim))
+            n_omega = 0
+            if len(t_hist) >= 3 and (t_hist[-1] - t_hist[-2]) < 0.9 * (t_hist[-2] - t_hist[-3]):
+                d_omega *= 2.0
+            if abs(d_t) < d_t_min:
+                stop_reason = "d_t_min"
+                break
+            if omega >= omega_max_stop:
+                stop_reason = "omega_max_stop"
+                break
+        if n_omega >= 5:
+            stop_reason = "omega_reduction_limit"
+            break
+        if len(t_hist) >= step_max:
+            stop_reason = "step_max"
+            break
+    torch.cuda.synchronize()
+    return LLResult(np.asarray(t_hist), np.asarray(omega_hist), np.asarray(umax_hist), U, stop_reason,
+                    perf_counter() - t0, stats.newton_iterations,
+                    solver.iteration_collector.get_total_iterations() - lin0)

@@ -23,6 +23,8 @@ SOURCES = [
     "constitutive.hip",
     "assembly.hip",
     "sparse.hip",
+    "p4_mfma.hip",
+    "mlp.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,198 @@
+// P4 tetrahedra: element tangent matrices on the FP64 matrix cores, then a deterministic row
+// gather into the fixed CSR pattern.
+//
+// ke = sum_q B_q^T (w_q D_q) B_q is one GEMM per element, ke (105 x 105) = A (105 x 144) * Bm (144 x 105)
+// with K = 24 points x 6 strains, A[i][k] = B_q[s][i] and Bm[k][j] = sum_t w_q D_q[s][t] B_q[t][j]
+// (k = 6 q + s).  One wavefront per element; the 105 x 105 result is padded to 7 x 7 tiles of
+// 16 x 16 and each tile accumulates 36 v_mfma_f64_16x16x4 steps.  Both operands are formed per lane
+// from the element's dphi and w*DS staged in LDS (B is sparse: every column of B_q has 3 nonzeros,
+// so an operand value costs <= 3 FMAs).  DS is symmetric for associated flow, so only the upper
+// tile triangle (28 of 49 tiles) is computed and mirrored on write.
+//
+// Stage 2 walks every row's slots in element order (the reference's `rows` order,
+// cython/assemble_tangent_values_3d.c:111-214) and adds ke[e][a][:] at the slot's CSR positions,
+// then applies the constrained diagonal, K_r blend and FP32 copy.
+//
+// Reference behaviour: cython/assemble_tangent_values_3d.c:13-109 (element matrices + scatter),
+// :111-214 (rows); flop count 2*105*105*144 per element.
+#include "ssx_common.h"
+
+namespace {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int NP4 = 35;
+constexpr int NLD4 = 105;
+constexpr int NQ4 = 24;
+constexpr int KDIM = NQ4 * 6;  // 144
+constexpr int TILES = 7;        // ceil(105 / 16)
+
+// Strain row s of B for local dof i = 3*node + d, gradients (gx, gy, gz) of that node.
+__device__ __forceinline__ double brow(int s, int d, double gx, double gy, double gz) {
+    switch (s) {
+        case 0: return d == 0 ? gx : 0.0;
+        case 1: return d == 1 ? gy : 0.0;
+        case 2: return d == 2 ? gz : 0.0;
+        case 3: return d == 0 ? gy : (d == 1 ? gx : 0.0);
+        case 4: return d == 1 ? gz : (d == 2 ? gy : 0.0);
+        default: return d == 0 ? gz : (d == 2 ? gx : 0.0);
+    }
+}
+
+__global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dphi, const double *__restrict__ DS,
+                                                   const double *__restrict__ w, int64_t n_elem, int64_t n_int,
+                                                   double *__restrict__ ke) {
+    __shared__ double sg[3][NQ4][NP4];  // gradients of the element's basis functions
+    __shared__ double sd[NQ4][36];      // w_q * DS_q (column-major 6x6)
+    const int64_t e = blockIdx.x;
+    if (e >= n_elem) return;
+    const int lane = threadIdx.x;
+    const int64_t g0 = e * NQ4;
+    for (int t = lane; t < 3 * NQ4 * NP4; t += 64) {
+        const int c = t / (NQ4 * NP4), rem = t % (NQ4 * NP4);
+        sg[c][rem / NP4][rem % NP4] = dphi[((int64_t)c * n_int + g0) * NP4 + rem];
+    }
+    for (int t = lane; t < NQ4 * 36; t += 64) {
+        const int q = t / 36;
+        sd[q][t % 36] = w[g0 + q] * DS[(g0 + q) * 36 + (t % 36)];
+    }
+    __syncthreads();
+
+    const int col = lane & 15;   // i for A, j for B and D
+    const int kq = lane >> 4;    // k offset within the 4-wide step
+    double *out = ke + e * (int64_t)(NLD4 * NLD4);
+    for (int ti = 0; ti < TILES; ++ti) {
+        d4 acc[TILES];
+#pragma unroll
+        for (int tj = 0; tj < TILES; ++tj) acc[tj] = d4{0.0, 0.0, 0.0, 0.0};
+        const int ia = 16 * ti + col;
+        const int na = ia / 3, da = ia % 3;
+        for (int ks = 0; ks < KDIM / 4; ++ks) {
+            const int k = 4 * ks + kq;
+            const int q = k / 6, s = k % 6;
+            const double a = ia < NLD4 ? brow(s, da, sg[0][q][na], sg[1][q][na], sg[2][q][na]) : 0.0;
+            const double *D = sd[q];
+#pragma unroll
+            for (int tj = 0; tj < TILES; ++tj) {
+                if (tj < ti) continue;
+                const int jb = 16 * tj + col;
+                double b = 0.0;
+                if (jb < NLD4) {
+                    const int nb = jb / 3, db = jb % 3;
+                    const double gx = sg[0][q][nb], gy = sg[1][q][nb], gz = sg[2][q][nb];
+                    // row s of (wD) B: sum_t wD[s][t] B[t][jb], D column-major D[s + 6 t]
+                    if (db == 0) b = D[s] * gx + D[s + 18] * gy + D[s + 30] * gz;
+                    else if (db == 1) b = D[s + 6] * gy + D[s + 18] * gx + D[s + 24] * gz;
+                    else b = D[s + 12] * gz + D[s + 24] * gy + D[s + 30] * gx;
+                }
+                acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[tj], 0, 0, 0);
+            }
+        }
+        // D layout: lane holds rows 4*(lane/16) + r of column lane%16
+#pragma unroll
+        for (int tj = 0; tj < TILES; ++tj) {
+            if (tj < ti) continue;
+            const int j = 16 * tj + col;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = 16 * ti + 4 * kq + r;
+                if (i < NLD4 && j < NLD4) {
+                    out[(int64_t)i * NLD4 + j] = acc[tj][r];
+                    if (tj > ti) out[(int64_t)j * NLD4 + i] = acc[tj][r];
+                }
+            }
+        }
+    }
+}
+
+// Row gather of element matrices: one lane per row, slots in element order; LDS row-block tile
+// as in the P2 tangent kernel (USE_LDS) or read-modify-write on the lane's own row in HBM.
+constexpr int kRowsKe = 64;
+
+template <bool USE_LDS>
+__global__ void __launch_bounds__(kRowsKe) k_scatter_ke_rows(const double *__restrict__ ke, int nld,
+                                                             const int32_t *__restrict__ row_slot_ptr,
+                                                             const int32_t *__restrict__ slot_elem,
+                                                             const uint8_t *__restrict__ slot_lrow,
+                                                             const int32_t *__restrict__ slot_pos,
+                                                             const int32_t *__restrict__ indptr,
+                                                             const uint8_t *__restrict__ row_free, int64_t n_rows,
+                                                             double r, const double *__restrict__ elastic,
+                                                             double *__restrict__ out, float *__restrict__ out32) {
+    extern __shared__ double lds[];
+    const int64_t r0 = (int64_t)blockIdx.x * kRowsKe;
+    const int64_t r1 = r0 + kRowsKe < n_rows ? r0 + kRowsKe : n_rows;
+    const int64_t v0 = indptr[r0];
+    const int64_t L = (int64_t)indptr[r1] - v0;
+    double *buf = USE_LDS ? lds : out + v0;
+    for (int64_t i = threadIdx.x; i < L; i += kRowsKe) buf[i] = 0.0;
+    __syncthreads();
+    const int64_t row = r0 + threadIdx.x;
+    if (row < r1) {
+        if (!row_free[row]) {
+            buf[indptr[row] - v0] = 1.0;
+        } else {
+            for (int32_t s = row_slot_ptr[row]; s < row_slot_ptr[row + 1]; ++s) {
+                const double *kr = ke + ((int64_t)slot_elem[s] * nld + slot_lrow[s]) * nld;
+                const int32_t *pos = slot_pos + (int64_t)s * nld;
+                for (int b = 0; b < nld; ++b) {
+                    const int32_t p = pos[b];
+                    if (p >= 0) buf[p - v0] += kr[b];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const double om = 1.0 - r;
+    for (int64_t i = threadIdx.x; i < L; i += kRowsKe) {
+        double v = buf[i];
+        if (elastic) v = om * v + r * elastic[v0 + i];
+        if (USE_LDS) out[v0 + i] = v;
+        else buf[i] = v;
+        if (out32) out32[v0 + i] = (float)v;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double *ke,
+                            void *stream) {
+    SSX_REQUIRE(n_elem >= 0, "ssx_element_matrices_p4: n_elem < 0");
+    if (n_elem == 0) return SSX_OK;
+    SSX_REQUIRE(dphi && DS && w && ke, "ssx_element_matrices_p4: null pointer");
+    hipLaunchKernelGGL(k_ke_p4_mfma, dim3((unsigned)n_elem), dim3(64), 0, ssx::as_stream(stream), dphi, DS, w, n_elem,
+                       n_elem * NQ4, ke);
+    SSX_CHECK_LAUNCH("ssx_element_matrices_p4");
+    return SSX_OK;
+}
+
+int ssx_scatter_element_rows(const double *ke, int nld, const int32_t *row_slot_ptr, const int32_t *slot_elem,
+                             const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr,
+                             const uint8_t *row_free, int64_t n_rows, double r, const double *elastic, double *out,
+                             float *out32, int64_t max_block_nnz, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && nld > 0, "ssx_scatter_element_rows: bad sizes");
+    if (n_rows == 0) return SSX_OK;
+    const unsigned grid = ssx::grid_for(n_rows, kRowsKe);
+    const size_t lds_bytes = (size_t)max_block_nnz * sizeof(double);
+    int dev = 0, max_lds = 65536;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+        max_lds = 65536;
+    bool lds_ok = max_block_nnz >= 0 && lds_bytes <= (size_t)max_lds;
+    if (lds_ok && lds_bytes > 65536)
+        lds_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scatter_ke_rows<true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
+    hipStream_t st = ssx::as_stream(stream);
+    if (lds_ok)
+        hipLaunchKernelGGL(k_scatter_ke_rows<true>, dim3(grid), dim3(kRowsKe), lds_bytes, st, ke, nld, row_slot_ptr,
+                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out, out32);
+    else
+        hipLaunchKernelGGL(k_scatter_ke_rows<false>, dim3(grid), dim3(kRowsKe), 0, st, ke, nld, row_slot_ptr,
+                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out, out32);
+    SSX_CHECK_LAUNCH("ssx_scatter_element_rows");
+    return SSX_OK;
+}
+
+}  // extern "C"

@@ -258,3 +258,44 @@ int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------------
+// Galerkin coarse operator for the p-multigrid: Ac = P^T A P accumulated into a dense column-major
+// (n_c x n_c) matrix.  P has at most a few entries per fine row (p_ptr/p_idx/p_w, CSR); one lane
+// per nonzero of A scatters its (<= 4) products with FP64 atomics.  Setup-time work of the
+// preconditioner (replaces the PETSc MatPtAP of linear/solver.py:510-513 feeding the coarse solve).
+// ---------------------------------------------------------------------------------------------
+namespace {
+__global__ void k_galerkin_dense(int64_t n_rows, const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+                                 const double *__restrict__ val, const int32_t *__restrict__ p_ptr,
+                                 const int32_t *__restrict__ p_idx, const double *__restrict__ p_w, int64_t n_c,
+                                 double *__restrict__ Ac) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
+    if (row >= n_rows) return;
+    const int32_t pi0 = p_ptr[row], pi1 = p_ptr[row + 1];
+    if (pi0 == pi1) return;
+    for (int32_t k = indptr[row] + threadIdx.x; k < indptr[row + 1]; k += blockDim.x) {
+        const int32_t col = indices[k];
+        const double a = val[k];
+        const int32_t pj0 = p_ptr[col], pj1 = p_ptr[col + 1];
+        for (int32_t s = pi0; s < pi1; ++s) {
+            const double ai = a * p_w[s];
+            const int64_t I = p_idx[s];
+            for (int32_t t = pj0; t < pj1; ++t)
+                atomicAdd(&Ac[(int64_t)p_idx[t] * n_c + I], ai * p_w[t]);
+        }
+    }
+}
+}  // namespace
+
+extern "C" int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                                  const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c,
+                                  double *Ac, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && n_c >= 0, "ssx_galerkin_dense: bad sizes");
+    if (n_rows == 0) return SSX_OK;
+    dim3 block(16, 16);
+    hipLaunchKernelGGL(k_galerkin_dense, dim3(ssx::grid_for(n_rows, 16)), block, 0, ssx::as_stream(stream), n_rows,
+                       indptr, indices, val, p_ptr, p_idx, p_w, n_c, Ac);
+    SSX_CHECK_LAUNCH("ssx_galerkin_dense");
+    return SSX_OK;
+}

@@ -139,8 +139,22 @@ class DevicePattern:
         _lib.call("ssx_force_gather", ptr(fe), ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow),
                   self.n_rows, self.nld, ptr(F), stream_ptr())
 
+    def element_matrices_p4(self, DS: torch.Tensor) -> torch.Tensor:
+        """(n_elem, 105, 105) element tangents of a P4 mesh on the FP64 matrix cores."""
+        if getattr(self, "_ke", None) is None:
+            self._ke = torch.empty((self.n_elem, self.nld, self.nld), dtype=torch.float64, device=DS.device)
+        _lib.call("ssx_element_matrices_p4", ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem, ptr(self._ke),
+                  stream_ptr())
+        return self._ke
+
     def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
                 elastic: torch.Tensor | None = None) -> None:
+        if self.dim == 3 and self.n_p == 35:
+            ke = self.element_matrices_p4(DS)
+            _lib.call("ssx_scatter_element_rows", ptr(ke), self.nld, ptr(self.row_slot_ptr), ptr(self.slot_elem),
+                      ptr(self.slot_lrow), ptr(self.slot_pos), ptr(self.indptr), ptr(self.row_free), self.n_rows,
+                      float(r), ptr(elastic), ptr(out), ptr(out32), self.max_block_nnz, stream_ptr())
+            return
         _lib.call("ssx_tangent_rows", self.dim, self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight),
                   ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow), ptr(self.slot_pos),
                   ptr(self.indptr), ptr(self.row_free), self.n_rows, self.n_elem, self.n_q, float(r),

@@ -1,0 +1,515 @@
+"""Device linear solvers: deflated flexible GMRES (FP64) with an FP32 or FP64 p-multigrid or
+Chebyshev-Jacobi preconditioner.
+
+Algorithm (linear/deflated_fgmres.py:174-295, the MATLAB-exact variant): coarse initial guess
+x0 = W (W^T b) from an A-orthonormal deflation basis W; each Arnoldi step applies the
+preconditioner, projects z <- z - W ((AW)^T z), multiplies by A and orthogonalises against the
+Krylov basis; the small least-squares problem on the Hessenberg matrix runs on the host; stop at
+relative residual < rtol (default 1e-1) or maxit (100).  The deflation basis follows
+linear/solver.py:2749-2852 (A-orthogonalised newest-first with the 1e-3 threshold of
+linear/orthogonalize.py:49-170, capped at 48 vectors, linear/solver.py:2288-2304).
+
+Differences, by design for the device: Gram-Schmidt is classical with one re-orthogonalisation
+pass (two batched dot launches per step instead of j+1 dependent ones); vectors are full-length
+(constrained dofs carry zeros and unit diagonal rows, so the free block is solved exactly as the
+reference's free-dof system).
+
+Preconditioners
+  * ``PMultigrid``: V-cycle P2 -> P1 (vertex dofs), Chebyshev(Jacobi) smoothing, Galerkin coarse
+    operator P^T A P formed on the device and factorised densely (Cholesky) -- in FP32 (values32)
+    or FP64.  Replaces Hypre BoomerAMG / the manual p-MG shell (linear/solver.py:96-779).
+  * ``ChebyshevJacobi``: polynomial preconditioner alone.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+from time import perf_counter
+
+import numpy as np
+import torch
+
+from . import _lib
+from .device import DeviceCSR, ptr, spmv_lanes, stream_ptr
+
+
+def _st():
+    return stream_ptr()
+
+
+# ---------------------------------------------------------------------------------------------
+# small device vector helpers (all through the ssx kernels)
+# ---------------------------------------------------------------------------------------------
+
+class _Scratch:
+    def __init__(self, dev):
+        self.part = torch.empty(1024, dtype=torch.float64, device=dev)
+        self.out = torch.empty(128, dtype=torch.float64, device=dev)
+
+
+def dot(x: torch.Tensor, y: torch.Tensor, scr: _Scratch) -> float:
+    _lib.call("ssx_dot", x.numel(), ptr(x), ptr(y), ptr(scr.part), ptr(scr.out), _st())
+    return float(scr.out[0].item())
+
+
+def axpby(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> None:
+    if y.dtype == torch.float64:
+        _lib.call("ssx_axpby_f64", y.numel(), float(a), ptr(x), float(b), ptr(y), _st())
+    else:
+        _lib.call("ssx_axpby_f32", y.numel(), float(a), ptr(x), float(b), ptr(y), _st())
+
+
+def spmv(A: DeviceCSR, x: torch.Tensor, y: torch.Tensor, alpha=1.0, beta=0.0, use32: bool = False) -> None:
+    if use32:
+        _lib.call("ssx_spmv_f32", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values32), ptr(x), ptr(y),
+                  float(alpha), float(beta), A.lanes, _st())
+    else:
+        _lib.call("ssx_spmv_f64", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(x), ptr(y),
+                  float(alpha), float(beta), A.lanes, _st())
+
+
+@dataclass
+class IterationCollector:
+    """Per-solve statistics (linear/collector.py:14-73)."""
+
+    iterations: list = field(default_factory=list)
+    solve_time: list = field(default_factory=list)
+    preconditioner_time: list = field(default_factory=list)
+    orthogonalization_time: list = field(default_factory=list)
+
+    def store(self, it, t_solve, t_pc, t_orth):
+        self.iterations.append(int(it))
+        self.solve_time.append(float(t_solve))
+        self.preconditioner_time.append(float(t_pc))
+        self.orthogonalization_time.append(float(t_orth))
+
+    def get_total_iterations(self):
+        return int(sum(self.iterations))
+
+    def get_total_solve_time(self):
+        return float(sum(self.solve_time))
+
+    def get_total_preconditioner_time(self):
+        return float(sum(self.preconditioner_time))
+
+    def get_total_orthogonalization_time(self):
+        return float(sum(self.orthogonalization_time))
+
+
+# ---------------------------------------------------------------------------------------------
+# Chebyshev smoother on one level
+# ---------------------------------------------------------------------------------------------
+
+class _Level:
+    """One multigrid level: CSR operator (FP32 or FP64 values), inverse diagonal, spectrum bound."""
+
+    def __init__(self, A: DeviceCSR, use32: bool):
+        self.A = A
+        self.use32 = use32
+        self.dtype = torch.float32 if use32 else torch.float64
+        dev = A.indptr.device
+        n = A.n_rows
+        self.n = n
+        self.dinv = torch.empty(n, dtype=self.dtype, device=dev)
+        self.r = torch.empty(n, dtype=self.dtype, device=dev)
+        self.d = torch.empty(n, dtype=self.dtype, device=dev)
+        self.x = torch.empty(n, dtype=self.dtype, device=dev)
+        self.lmax = 2.0
+
+    def refresh(self, power_its: int = 12, seed: int = 0):
+        A = self.A
+        fn = "ssx_inv_diag_f32" if self.use32 else "ssx_inv_diag_f64"
+        vals = A.values32 if self.use32 else A.values
+        _lib.call(fn, A.n_rows, 0, ptr(A.indptr), ptr(A.indices), ptr(vals), ptr(self.dinv), _st())
+        # power iteration on D^-1 A
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        v = torch.rand(self.n, generator=g, dtype=torch.float64).to(self.dinv.device).to(self.dtype)
+        v /= torch.linalg.vector_norm(v)
+        lam = 1.0
+        for _ in range(power_its):
+            spmv(A, v, self.r, use32=self.use32)
+            self.r.mul_(self.dinv)
+            lam = float(torch.linalg.vector_norm(self.r).item())
+            if lam == 0.0:
+                break
+            v = self.r / lam
+        self.lmax = 1.1 * lam
+
+    def cheb(self, b: torch.Tensor, x: torch.Tensor, degree: int, lo_frac: float, zero_guess: bool):
+        """`degree` Chebyshev(Jacobi) steps on A x = b over [lo_frac*lmax, lmax]."""
+        beta = self.lmax
+        alpha = lo_frac * self.lmax
+        theta = 0.5 * (beta + alpha)
+        delta = 0.5 * (beta - alpha)
+        sigma = theta / delta
+        rho = 1.0 / sigma
+        upd = "ssx_cheb_update_f32" if self.use32 else "ssx_cheb_update_f64"
+        # r = b - A x
+        axpby(1.0, b, 0.0, self.r)
+        if not zero_guess:
+            spmv(self.A, x, self.r, alpha=-1.0, beta=1.0, use32=self.use32)
+        else:
+            x.zero_()
+        self.d.zero_()
+        _lib.call(upd, self.n, ptr(self.dinv), ptr(self.r), 1.0 / theta, 0.0, ptr(x), ptr(self.d), _st())
+        for _ in range(degree - 1):
+            rho_new = 1.0 / (2.0 * sigma - rho)
+            axpby(1.0, b, 0.0, self.r)
+            spmv(self.A, x, self.r, alpha=-1.0, beta=1.0, use32=self.use32)
+            _lib.call(upd, self.n, ptr(self.dinv), ptr(self.r), 2.0 * rho_new / delta, rho_new * rho, ptr(x),
+                      ptr(self.d), _st())
+            rho = rho_new
+
+
+class ChebyshevJacobi:
+    """Polynomial preconditioner: `degree` Chebyshev(Jacobi) steps from a zero guess."""
+
+    def __init__(self, degree: int = 8, use32: bool = True, lo_frac: float = 0.02):
+        self.degree = degree
+        self.use32 = use32
+        self.lo_frac = lo_frac
+        self.level = None
+
+    def setup(self, A: DeviceCSR, mesh_info=None):
+        if self.level is None or self.level.A is not A:
+            self.level = _Level(A, self.use32)
+        self.level.refresh()
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        L = self.level
+        if self.use32:
+            b = torch.empty(L.n, dtype=torch.float32, device=r.device)
+            _lib.call("ssx_cast_f64_f32", L.n, ptr(r), ptr(b), _st())
+            L.cheb(b, L.x, self.degree, self.lo_frac, zero_guess=True)
+            _lib.call("ssx_cast_f32_f64", L.n, ptr(L.x), ptr(z), _st())
+        else:
+            L.cheb(r, z, self.degree, self.lo_frac, zero_guess=True)
+
+
+# ---------------------------------------------------------------------------------------------
+# p-multigrid P2 -> P1
+# ---------------------------------------------------------------------------------------------
+
+_EDGES = {3: ((0, 1), (1, 2), (0, 2), (1, 3), (2, 3), (0, 3)), 2: ((1, 2), (0, 2), (0, 1))}
+
+
+def p2_to_p1_prolongation(elem: np.ndarray, q_mask: np.ndarray):
+    """Rows of P (fine dof -> coarse dof, weight) for the P2 -> P1 interpolation on free dofs.
+
+    Vertex nodes inject, edge-midpoint nodes average their two vertices (the P1 basis evaluated at
+    the P2 nodes, linear/pmg.py:365-436).  Constrained fine dofs get empty rows; constrained coarse
+    dofs are left out of P (they carry an identity block in the coarse operator)."""
+    elem = np.asarray(elem, dtype=np.int64)
+    dim, n_nodes = q_mask.shape
+    nv = dim + 1
+    verts = np.unique(elem[:nv].reshape(-1))
+    cnum = np.full(n_nodes, -1, dtype=np.int64)
+    cnum[verts] = np.arange(verts.size)
+    parents = {}
+    for k, (a, b) in enumerate(_EDGES[dim]):
+        mids = elem[nv + k]
+        for m, va, vb in zip(mids, elem[a], elem[b]):
+            parents[int(m)] = (int(va), int(vb))
+    free = q_mask.T.reshape(-1)
+    n_c_nodes = verts.size
+    rows_ptr = np.zeros(dim * n_nodes + 1, dtype=np.int64)
+    idx, w = [], []
+    for node in range(n_nodes):
+        for c in range(dim):
+            f = dim * node + c
+            ent = []
+            if free[f]:
+                if cnum[node] >= 0:
+                    ent = [(dim * cnum[node] + c, 1.0)]
+                elif node in parents:
+                    va, vb = parents[node]
+                    ent = [(dim * cnum[va] + c, 0.5), (dim * cnum[vb] + c, 0.5)]
+            for (J, wt) in ent:
+                if free[dim * verts[J // dim] + c]:
+                    idx.append(J)
+                    w.append(wt)
+            rows_ptr[f + 1] = len(idx)
+    c_free = np.zeros(dim * n_c_nodes, dtype=bool)
+    c_free[(dim * np.arange(n_c_nodes)[:, None] + np.arange(dim)).reshape(-1)] = free[
+        (dim * verts[:, None] + np.arange(dim)).reshape(-1)]
+    return rows_ptr, np.asarray(idx, dtype=np.int64), np.asarray(w, dtype=np.float64), dim * n_c_nodes, c_free
+
+
+class PMultigrid:
+    """Two-level V-cycle: Chebyshev(Jacobi) pre/post smoothing on the P2 operator, Galerkin coarse
+    operator on the P1 vertex dofs solved by a dense Cholesky factorisation."""
+
+    def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
+                 coarse_dtype=torch.float64):
+        self.use32 = use32
+        self.degree = degree
+        self.lo_frac = lo_frac
+        self.coarse_dtype = coarse_dtype
+        p_ptr, p_idx, p_w, n_c, c_free = p2_to_p1_prolongation(elem, q_mask)
+        self.n_c = n_c
+        self.c_free = c_free
+        self._p_host = (p_ptr, p_idx, p_w)
+        self.level = None
+        self._dev_ready = False
+
+    def _to_device(self, dev, n_f):
+        p_ptr, p_idx, p_w = self._p_host
+        import scipy.sparse as sp
+        P = sp.csr_matrix((p_w, p_idx, p_ptr), shape=(n_f, self.n_c))
+        PT = P.T.tocsr()
+        mk = lambda M: (torch.from_numpy(M.indptr.astype(np.int32)).to(dev),  # noqa: E731
+                        torch.from_numpy(M.indices.astype(np.int32)).to(dev),
+                        torch.from_numpy(M.data.astype(np.float64)).to(dev))
+        self.P = DeviceCSR(n_f, *mk(P), None, spmv_lanes(P.indptr))
+        self.P.values32 = self.P.values.to(torch.float32)
+        self.PT = DeviceCSR(self.n_c, *mk(PT), None, spmv_lanes(PT.indptr))
+        self.PT.values32 = self.PT.values.to(torch.float32)
+        self.p_ptr = torch.from_numpy(p_ptr.astype(np.int32)).to(dev)
+        self.p_idx = torch.from_numpy(p_idx.astype(np.int32)).to(dev)
+        self.p_w = torch.from_numpy(p_w).to(dev)
+        self.cfix = torch.from_numpy(np.flatnonzero(~self.c_free)).to(dev)
+        dt = torch.float32 if self.use32 else torch.float64
+        self.rc = torch.empty(self.n_c, dtype=dt, device=dev)
+        self.xc = torch.empty(self.n_c, dtype=dt, device=dev)
+        self._dev_ready = True
+
+    def setup(self, A: DeviceCSR, mesh_info=None):
+        dev = A.indptr.device
+        if not self._dev_ready:
+            self._to_device(dev, A.n_rows)
+        if self.level is None or self.level.A is not A:
+            self.level = _Level(A, self.use32)
+        self.level.refresh()
+        # coarse operator (FP64 accumulate), identity on constrained coarse dofs
+        Ac = torch.zeros((self.n_c, self.n_c), dtype=torch.float64, device=dev)
+        _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.p_ptr),
+                  ptr(self.p_idx), ptr(self.p_w), self.n_c, ptr(Ac), _st())
+        Ac[self.cfix, self.cfix] = 1.0
+        Ac = 0.5 * (Ac + Ac.t())
+        L, info = torch.linalg.cholesky_ex(Ac.to(self.coarse_dtype))
+        if int(info.item()) != 0:
+            # indefinite coarse operator (strong softening): fall back to a shifted factorisation
+            shift = 1e-8 * float(Ac.diagonal().abs().max().item())
+            L = torch.linalg.cholesky(Ac.to(self.coarse_dtype) + shift * torch.eye(self.n_c, device=dev,
+                                                                                   dtype=self.coarse_dtype))
+        self.Lc = L
+        del Ac
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        Lv = self.level
+        n = Lv.n
+        if self.use32:
+            b = torch.empty(n, dtype=torch.float32, device=r.device)
+            _lib.call("ssx_cast_f64_f32", n, ptr(r), ptr(b), _st())
+        else:
+            b = r
+        x = Lv.x
+        Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=True)          # pre-smooth
+        res = torch.empty_like(x)
+        axpby(1.0, b, 0.0, res)
+        spmv(Lv.A, x, res, alpha=-1.0, beta=1.0, use32=self.use32)          # res = b - A x
+        spmv(self.PT, res, self.rc, use32=self.use32)                          # restrict
+        ec = torch.cholesky_solve(self.rc.to(self.coarse_dtype)[:, None], self.Lc)[:, 0]
+        self.xc.copy_(ec.to(self.xc.dtype))
+        spmv(self.P, self.xc, x, alpha=1.0, beta=1.0, use32=self.use32)       # prolong + correct
+        Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=False)          # post-smooth
+        if self.use32:
+            _lib.call("ssx_cast_f32_f64", n, ptr(x), ptr(z), _st())
+        else:
+            z.copy_(x)
+
+
+# ---------------------------------------------------------------------------------------------
+# Deflated FGMRES with the reference's solver duck API
+# ---------------------------------------------------------------------------------------------
+
+class DeflatedFGMRES:
+    def __init__(self, preconditioner, *, tolerance: float = 1e-1, max_iterations: int = 100,
+                 tolerance_deflation_basis: float = 1e-3, max_deflation_basis_vectors: int = 48,
+                 reorth_passes: int = 1):
+        self.pc = preconditioner
+        self.tolerance = float(tolerance)
+        self.max_iterations = int(max_iterations)
+        self.tolerance_deflation_basis = float(tolerance_deflation_basis)
+        self.max_basis = int(max_deflation_basis_vectors)
+        self.reorth_passes = int(reorth_passes)
+        self.iteration_collector = IterationCollector()
+        self.deflation_basis: torch.Tensor | None = None   # (k, n) raw vectors, newest last
+        self._W = None      # A-orthonormal basis (k, n)
+        self._AW = None
+        self._A = None
+        self._scr = None
+        self.last_residual_history = None
+
+    # duck API --------------------------------------------------------------------------------
+    def prefers_full_system_operator(self):
+        return True
+
+    def supports_a_orthogonalization(self):
+        return True
+
+    def supports_dynamic_deflation_basis(self):
+        return True
+
+    def preconditioner_requires_explicit_matrix(self):
+        return False
+
+    def needs_preconditioning_matrix_refresh(self):
+        return False
+
+    def release_iteration_resources(self):
+        pass
+
+    def notify_continuation_attempt(self, success: bool = True):
+        pass
+
+    def copy(self):
+        c = DeflatedFGMRES(self.pc, tolerance=self.tolerance, max_iterations=self.max_iterations,
+                           tolerance_deflation_basis=self.tolerance_deflation_basis,
+                           max_deflation_basis_vectors=self.max_basis, reorth_passes=self.reorth_passes)
+        c.deflation_basis = self.deflation_basis
+        c.iteration_collector = self.iteration_collector
+        return c
+
+    def get_deflation_basis_snapshot(self):
+        return None if self.deflation_basis is None else self.deflation_basis.clone()
+
+    def restore_deflation_basis(self, snap):
+        self.deflation_basis = None if snap is None else snap.clone()
+        self._W = None
+
+    def expand_deflation_basis(self, v: torch.Tensor):
+        v = v.reshape(1, -1).to(torch.float64)
+        if self.deflation_basis is None:
+            self.deflation_basis = v.clone()
+        else:
+            self.deflation_basis = torch.cat([self.deflation_basis, v], dim=0)
+        if self.deflation_basis.shape[0] > self.max_basis:
+            self.deflation_basis = self.deflation_basis[-self.max_basis:].contiguous()
+        self._W = None
+
+    def setup_preconditioner(self, A: DeviceCSR, full_matrix=None, free_indices=None, preconditioning_matrix=None):
+        t0 = perf_counter()
+        self._A = A
+        if self._scr is None:
+            self._scr = _Scratch(A.indptr.device)
+        self.pc.setup(A)
+        self._pc_setup_time = perf_counter() - t0
+
+    def A_orthogonalize(self, A: DeviceCSR):
+        """A-orthonormalise the basis newest-first, dropping columns with |v^T A v| <= eps
+        (linear/orthogonalize.py:49-170)."""
+        self._A = A
+        B = self.deflation_basis
+        if B is None or B.shape[0] == 0:
+            self._W = None
+            self._AW = None
+            return
+        dev = B.device
+        kmax, n = B.shape
+        Wbuf = torch.empty((kmax, n), dtype=torch.float64, device=dev)
+        av = torch.empty(n, dtype=torch.float64, device=dev)
+        coeff = torch.empty(kmax, dtype=torch.float64, device=dev)
+        scr = self._scr if self._scr is not None else _Scratch(dev)
+        signs: list[float] = []
+        cnt = 0
+        for idx in range(kmax - 1, -1, -1):
+            v = B[idx].clone()
+            spmv(A, v, av)
+            if cnt:
+                _lib.call("ssx_dot_cols", n, cnt, ptr(Wbuf), n, ptr(av), ptr(coeff), _st())
+                c = coeff[:cnt].cpu().numpy() * np.asarray(signs)
+                coeff[:cnt].copy_(torch.from_numpy(c))
+                _lib.call("ssx_gemv_cols", n, cnt, ptr(Wbuf), n, ptr(coeff), -1.0, ptr(v), _st())
+            nv = dot(v, av, scr)
+            if abs(nv) > self.tolerance_deflation_basis:
+                axpby(1.0 / np.sqrt(abs(nv)), v, 0.0, Wbuf[cnt])
+                signs.append(1.0 if nv > 0 else -1.0)
+                cnt += 1
+        if cnt == 0:
+            self._W = None
+            self._AW = None
+            return
+        Ws = [Wbuf[i] for i in range(cnt)]
+        W = torch.stack(Ws[::-1]).contiguous()
+        AW = torch.empty_like(W)
+        for j in range(W.shape[0]):
+            spmv(A, W[j], AW[j])
+        self._W, self._AW = W, AW
+
+    def solve(self, A: DeviceCSR, b: torch.Tensor, full_rhs=None, local_rhs=None, free_indices=None):
+        t0 = perf_counter()
+        if self._scr is None:
+            self._scr = _Scratch(A.indptr.device)
+        scr = self._scr
+        dev = b.device
+        n = b.numel()
+        m = self.max_iterations
+        tol = self.tolerance
+        t_pc = 0.0
+        t_orth = 0.0
+        W, AW = self._W, self._AW
+        k = 0 if W is None else W.shape[0]
+        x = torch.zeros(n, dtype=torch.float64, device=dev)
+        if k:
+            h = torch.empty(k, dtype=torch.float64, device=dev)
+            _lib.call("ssx_dot_cols", n, k, ptr(W), n, ptr(b), ptr(h), _st())
+            _lib.call("ssx_gemv_cols", n, k, ptr(W), n, ptr(h), 1.0, ptr(x), _st())
+        r = b.clone()
+        spmv(A, x, r, alpha=-1.0, beta=1.0)
+        bnorm = np.sqrt(max(dot(b, b, scr), 0.0)) or 1.0
+        rnorm = np.sqrt(max(dot(r, r, scr), 0.0))
+        hist = [rnorm / bnorm]
+        if hist[0] < tol or rnorm == 0.0:
+            self.iteration_collector.store(0, perf_counter() - t0, 0.0, 0.0)
+            self.last_residual_history = np.array(hist)
+            return x
+        V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
+        Z = torch.empty((m, n), dtype=torch.float64, device=dev)
+        H = np.zeros((m + 1, m))
+        g = np.zeros(m + 1)
+        g[0] = rnorm
+        axpby(1.0 / rnorm, r, 0.0, V[0])
+        hbuf = torch.empty(m + 1, dtype=torch.float64, device=dev)
+        hbuf2 = torch.empty(m + 1, dtype=torch.float64, device=dev)
+        cbuf = torch.empty(max(k, 1), dtype=torch.float64, device=dev)
+        iters = 0
+        y = np.zeros(0)
+        for j in range(m):
+            tp = perf_counter()
+            self.pc.apply(V[j], Z[j])
+            t_pc += perf_counter() - tp
+            to = perf_counter()
+            if k:
+                _lib.call("ssx_dot_cols", n, k, ptr(AW), n, ptr(Z[j]), ptr(cbuf), _st())
+                _lib.call("ssx_gemv_cols", n, k, ptr(W), n, ptr(cbuf), -1.0, ptr(Z[j]), _st())
+            u = V[j + 1]
+            spmv(A, Z[j], u)
+            _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(hbuf), _st())
+            _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf), -1.0, ptr(u), _st())
+            for _ in range(self.reorth_passes):
+                _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(hbuf2), _st())
+                _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf2), -1.0, ptr(u), _st())
+                hbuf[: j + 1] += hbuf2[: j + 1]
+            hn = np.sqrt(max(dot(u, u, scr), 0.0))
+            H[: j + 1, j] = hbuf[: j + 1].cpu().numpy()
+            H[j + 1, j] = hn
+            t_orth += perf_counter() - to
+            iters = j + 1
+            if hn < 1e-14:
+                y, *_ = np.linalg.lstsq(H[: j + 1, : j + 1], g[: j + 1], rcond=None)
+                break
+            axpby(1.0 / hn, u, 0.0, u)
+            y, *_ = np.linalg.lstsq(H[: j + 2, : j + 1], g[: j + 2], rcond=None)
+            res = float(np.linalg.norm(g[: j + 2] - H[: j + 2, : j + 1] @ y)) / bnorm
+            hist.append(res)
+            if not np.isfinite(res):
+                break
+            if res < tol:
+                break
+        if iters:
+            yt = torch.from_numpy(np.ascontiguousarray(y[:iters])).to(dev)
+            _lib.call("ssx_gemv_cols", n, iters, ptr(Z), n, ptr(yt), 1.0, ptr(x), _st())
+        self.last_residual_history = np.array(hist)
+        self.iteration_collector.store(iters, perf_counter() - t0, t_pc, t_orth)
+        return x

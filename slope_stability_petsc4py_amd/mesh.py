@@ -126,6 +126,79 @@ def elevate_tet4_to_tet10(coord: np.ndarray, elem4: np.ndarray) -> tuple[np.ndar
     return coord10, elem10
 
 
+def elevate_tet4_to_tet35(coord: np.ndarray, elem4: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    """Order-4 tetrahedra (io.py:339-398): per edge 3 nodes at (1-k/4) v_lo + (k/4) v_hi (k = 1..3,
+    reversed when the element walks the edge from its larger vertex), per face the 3 interior
+    nodes (2,1,1), (1,2,1), (1,1,2) in the face's sorted-vertex frame, one centroid per element;
+    local order: vertices, edges (01,12,02,13,23,03), faces (012,013,023,123), centroid.
+    New nodes are numbered edges, faces, centroids (the caller reorders xyz)."""
+    elem4 = np.asarray(elem4, dtype=np.int64)
+    n_v = coord.shape[1]
+    n_e = elem4.shape[1]
+    # edges
+    a = np.stack([elem4[i] for i, _ in _TET_EDGES], axis=1)
+    b = np.stack([elem4[j] for _, j in _TET_EDGES], axis=1)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    key = (lo * n_v + hi).ravel()
+    uniq, first, inv = np.unique(key, return_index=True, return_inverse=True)
+    order = np.argsort(first, kind="stable")
+    rank = np.empty_like(order)
+    rank[order] = np.arange(order.size)
+    eid = rank[inv].reshape(n_e, 6)
+    lo_u, hi_u = lo.ravel()[first[order]], hi.ravel()[first[order]]
+    n_edges = lo_u.size
+    pts = [np.empty((3, 0))]
+    for k in (1, 2, 3):
+        pts.append(((4 - k) / 4.0) * coord[:, lo_u] + (k / 4.0) * coord[:, hi_u])
+    edge_nodes = n_v + np.stack([np.arange(n_edges) * 3 + t for t in range(3)], axis=1)  # (n_edges, 3) lo->hi
+    edge_pts = np.stack(pts[1:], axis=2).reshape(3, -1)  # node (edge, k) -> column edge*3 + k-1
+    fwd = (a == lo)  # element walks lo -> hi
+    loc_edges = []
+    for t in range(6):
+        en = edge_nodes[eid[:, t]]
+        loc_edges.append(np.where(fwd[:, t:t + 1], en, en[:, ::-1]))
+    # faces
+    faces = ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3))
+    fv = np.stack([np.stack([elem4[i] for i in f], axis=1) for f in faces], axis=1)  # (n_e, 4, 3) local verts
+    can = np.sort(fv, axis=2)
+    fkey = (can[..., 0] * n_v + can[..., 1]) * n_v + can[..., 2]
+    fu, ffirst, finv = np.unique(fkey.ravel(), return_index=True, return_inverse=True)
+    forder = np.argsort(ffirst, kind="stable")
+    frank = np.empty_like(forder)
+    frank[forder] = np.arange(forder.size)
+    fid = frank[finv].reshape(n_e, 4)
+    can_u = can.reshape(-1, 3)[ffirst[forder]]  # (n_faces, 3)
+    n_faces = can_u.shape[0]
+    tri_t = ((2, 1, 1), (1, 2, 1), (1, 1, 2))
+    base_f = n_v + 3 * n_edges
+    face_pts = []
+    for t in tri_t:
+        p = np.zeros((3, n_faces))
+        for cnt, col in zip(t, range(3)):
+            p = p + (cnt / 4.0) * coord[:, can_u[:, col]]
+        face_pts.append(p)
+    face_pts = np.stack(face_pts, axis=2).reshape(3, -1)  # (face, t) -> column face*3 + t
+    loc_faces = []
+    for fl in range(4):
+        verts = fv[:, fl, :]                              # local order
+        cf = can[:, fl, :]
+        pos_in_can = (verts[:, :, None] == cf[:, None, :]).argmax(axis=2)  # local idx -> canonical idx
+        nodes = []
+        for t in tri_t:
+            cc = np.zeros((n_e, 3), dtype=np.int64)
+            for li in range(3):
+                cc[np.arange(n_e), pos_in_can[:, li]] = t[li]
+            tidx = np.where((cc == np.array([2, 1, 1])).all(1), 0, np.where((cc == np.array([1, 2, 1])).all(1), 1, 2))
+            nodes.append(base_f + 3 * fid[:, fl] + tidx)
+        loc_faces.append(np.stack(nodes, axis=1))
+    cen = 0.25 * (coord[:, elem4[0]] + coord[:, elem4[1]] + coord[:, elem4[2]] + coord[:, elem4[3]])
+    base_c = base_f + 3 * n_faces
+    elem35 = np.vstack([elem4] + [e.T for e in loc_edges] + [f.T for f in loc_faces]
+                       + [(base_c + np.arange(n_e))[None]])
+    coord35 = np.hstack((coord, edge_pts, face_pts, cen))
+    return coord35, elem35
+
+
 def xyz_order(coord: np.ndarray) -> np.ndarray:
     keys = (coord[1], coord[0]) if coord.shape[0] == 2 else (coord[2], coord[1], coord[0])
     return np.lexsort(keys).astype(np.int64)
@@ -206,8 +279,10 @@ def slope_3d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.
         coord_e, elem = coord, elem4
     elif elem_type.upper() == "P2":
         coord_e, elem = elevate_tet4_to_tet10(coord, elem4)
+    elif elem_type.upper() == "P4":
+        coord_e, elem = elevate_tet4_to_tet35(coord, elem4)
     else:
-        raise ValueError("slope_3d supports P1 and P2")
+        raise ValueError("slope_3d supports P1, P2 and P4")
     q = np.ones((3, coord_e.shape[1]), dtype=bool)
     tol = 1e-9
     xmax, zmax = coord_e[0].max(), coord_e[2].max()
@@ -232,42 +307,139 @@ def slope_3d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.
 
 def read_gmsh22(path: str | Path):
     """Return (coord (3, n), tet4 (4, n_t), tet_phys (n_t,), tri3 (3, n_f), tri_phys (n_f,)) with
-    0-based node numbers following the file's node order."""
-    lines = Path(path).read_text().splitlines()
-    it = iter(range(len(lines)))
-    coord = None
-    node_index = None
+    0-based node numbers following the file's node order.  Reads ASCII and binary 2.2 files."""
+    data = Path(path).read_bytes()
+    pos = data.index(b"$MeshFormat") + len(b"$MeshFormat\n")
+    line_end = data.index(b"\n", pos)
+    version, ftype, _dsize = data[pos:line_end].split()
+    binary = int(ftype) == 1
+    pos = line_end + 1
+    if binary:
+        pos += 4  # endianness marker (int 1)
+    nodes_at = data.index(b"$Nodes", pos) + len(b"$Nodes\n")
+    line_end = data.index(b"\n", nodes_at)
+    n_nodes = int(data[nodes_at:line_end])
+    pos = line_end + 1
+    if binary:
+        rec = np.dtype([("id", "<i4"), ("xyz", "<f8", 3)])
+        arr = np.frombuffer(data, dtype=rec, count=n_nodes, offset=pos)
+        ids, xyz = arr["id"].astype(np.int64), arr["xyz"]
+    else:
+        txt = data[pos:data.index(b"$EndNodes", pos)].split()
+        a = np.array(txt, dtype=np.float64).reshape(n_nodes, 4)
+        ids, xyz = a[:, 0].astype(np.int64), a[:, 1:4]
+    node_index = np.full(ids.max() + 1, -1, dtype=np.int64)
+    node_index[ids] = np.arange(n_nodes)
+    el_at = data.index(b"$Elements", pos) + len(b"$Elements\n")
+    line_end = data.index(b"\n", el_at)
+    n_el = int(data[el_at:line_end])
+    pos = line_end + 1
+    nn = {1: 2, 2: 3, 4: 4, 15: 1}
     tets, tet_tag, tris, tri_tag = [], [], [], []
-    i = 0
-    while i < len(lines):
-        s = lines[i].strip()
-        if s == "$Nodes":
-            n = int(lines[i + 1])
-            arr = np.array([lines[i + 2 + k].split() for k in range(n)], dtype=np.float64)
-            ids = arr[:, 0].astype(np.int64)
-            coord = arr[:, 1:4].T.copy()
-            node_index = np.full(ids.max() + 1, -1, dtype=np.int64)
-            node_index[ids] = np.arange(n)
-            i += n + 2
-            continue
-        if s == "$Elements":
-            n = int(lines[i + 1])
-            for k in range(n):
-                parts = lines[i + 2 + k].split()
-                etype, ntags = int(parts[1]), int(parts[2])
-                tags = parts[3:3 + ntags]
-                nodes = [int(v) for v in parts[3 + ntags:]]
-                phys = int(tags[0]) if ntags else 0
-                if etype == 4:
-                    tets.append(nodes)
-                    tet_tag.append(phys)
-                elif etype == 2:
-                    tris.append(nodes)
-                    tri_tag.append(phys)
-            i += n + 2
-            continue
-        i += 1
-    del it
-    tet4 = node_index[np.asarray(tets, dtype=np.int64)].T
-    tri3 = node_index[np.asarray(tris, dtype=np.int64)].T if tris else np.empty((3, 0), dtype=np.int64)
-    return coord, tet4, np.asarray(tet_tag, dtype=np.int64), tri3, np.asarray(tri_tag, dtype=np.int64)
+    if binary:
+        done = 0
+        while done < n_el:
+            etype, nfollow, ntags = np.frombuffer(data, dtype="<i4", count=3, offset=pos)
+            pos += 12
+            width = 1 + int(ntags) + nn[int(etype)]
+            blk = np.frombuffer(data, dtype="<i4", count=int(nfollow) * width, offset=pos).reshape(int(nfollow), width)
+            pos += 4 * blk.size
+            phys = blk[:, 1] if ntags else np.zeros(int(nfollow), dtype=np.int32)
+            conn = blk[:, 1 + int(ntags):]
+            if etype == 4:
+                tets.append(conn)
+                tet_tag.append(phys)
+            elif etype == 2:
+                tris.append(conn)
+                tri_tag.append(phys)
+            done += int(nfollow)
+    else:
+        lines = data[pos:data.index(b"$EndElements", pos)].decode().splitlines()
+        for ln in lines[:n_el]:
+            parts = [int(v) for v in ln.split()]
+            etype, ntags = parts[1], parts[2]
+            phys = parts[3] if ntags else 0
+            conn = parts[3 + ntags:]
+            if etype == 4:
+                tets.append(np.array([conn]))
+                tet_tag.append(np.array([phys]))
+            elif etype == 2:
+                tris.append(np.array([conn]))
+                tri_tag.append(np.array([phys]))
+    tet4 = node_index[np.concatenate(tets).astype(np.int64)].T
+    tri3 = node_index[np.concatenate(tris).astype(np.int64)].T if tris else np.empty((3, 0), dtype=np.int64)
+    return (xyz.T.copy(), tet4, np.concatenate(tet_tag).astype(np.int64),
+            tri3, np.concatenate(tri_tag).astype(np.int64) if tri_tag else np.empty(0, dtype=np.int64))
+
+
+_P4_FACE_LOCAL = {  # local P4 node indices of each tetrahedron face (vertices, edge nodes, face nodes)
+    (0, 1, 2): [0, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 22, 23, 24],
+    (0, 1, 3): [0, 1, 3, 4, 5, 6, 13, 14, 15, 19, 20, 21, 25, 26, 27],
+    (0, 2, 3): [0, 2, 3, 10, 11, 12, 16, 17, 18, 19, 20, 21, 28, 29, 30],
+    (1, 2, 3): [1, 2, 3, 7, 8, 9, 13, 14, 15, 16, 17, 18, 31, 32, 33],
+}
+
+
+def _p4_face_nodes(tri3, tet4, elem35, n_v):
+    """(15, n_faces) P4 nodes of the boundary triangles, found through an element that owns each face."""
+    key_of = lambda a, b, c: (a * n_v + b) * n_v + c  # noqa: E731
+    lut = {}
+    for f, loc in _P4_FACE_LOCAL.items():
+        v = np.sort(tet4[list(f)], axis=0)
+        k = key_of(v[0], v[1], v[2])
+        for e, kk in enumerate(k.tolist()):
+            lut.setdefault(kk, (e, loc))
+    t = np.sort(tri3, axis=0)
+    out = np.empty((15, tri3.shape[1]), dtype=np.int64)
+    for i, kk in enumerate(key_of(t[0], t[1], t[2]).tolist()):
+        e, loc = lut[kk]
+        out[:, i] = elem35[loc, e]
+    return out
+
+
+def tet_mesh_from_arrays(coord, tet4, material, tri3, boundary, dirichlet_labels: dict, elem_type: str = "P2",
+                         reorder: bool = True) -> SlopeMesh:
+    """Gmsh-family slope: elevate tet4 (io.py:235-276), build the Dirichlet mask from labelled
+    boundary triangles (io.py:61-93: each axis constrained on its label set; triangle midpoints
+    inherit the constraint through the elevated faces), reorder xyz (mesh/reorder.py:49-59)."""
+    coord = np.asarray(coord, dtype=np.float64)
+    if elem_type.upper() == "P2":
+        coord_e, elem = elevate_tet4_to_tet10(coord, tet4)
+        # face midpoint nodes of the boundary triangles: look the edges up through the element edges
+        n_v = coord.shape[1]
+        e4 = np.asarray(tet4, dtype=np.int64)
+        a = np.stack([e4[i] for i, _ in _TET_EDGES], axis=1).ravel()
+        b = np.stack([e4[j] for _, j in _TET_EDGES], axis=1).ravel()
+        keys = np.minimum(a, b) * n_v + np.maximum(a, b)
+        mids = elem[4:].T.ravel()
+        order = np.argsort(keys)
+        ks, ms = keys[order], mids[order]
+        tri = np.asarray(tri3, dtype=np.int64)
+        tri_edges = [(0, 1), (1, 2), (0, 2)]
+        surf_nodes = [tri]
+        for i, j in tri_edges:
+            kk = np.minimum(tri[i], tri[j]) * n_v + np.maximum(tri[i], tri[j])
+            surf_nodes.append(ms[np.searchsorted(ks, kk)][None])
+        surf = np.vstack(surf_nodes)
+    elif elem_type.upper() == "P1":
+        coord_e, elem, surf = coord, np.asarray(tet4, dtype=np.int64), np.asarray(tri3, dtype=np.int64)
+    elif elem_type.upper() == "P4":
+        coord_e, elem = elevate_tet4_to_tet35(coord, tet4)
+        # boundary faces: every P4 node of a labelled triangle (vertices, edge and face nodes) is
+        # found through the elements' faces
+        surf = _p4_face_nodes(np.asarray(tri3, dtype=np.int64), np.asarray(tet4, dtype=np.int64), elem,
+                              coord.shape[1])
+    else:
+        raise ValueError("P1, P2 and P4 supported")
+    q = np.ones((3, coord_e.shape[1]), dtype=bool)
+    labels = np.asarray(boundary, dtype=np.int64)
+    for axis, name in enumerate("xyz"):
+        lab = dirichlet_labels.get(name, ())
+        if lab:
+            sel = np.isin(labels, np.asarray(lab))
+            q[axis, surf[:, sel].ravel()] = False
+    mesh = SlopeMesh(coord=coord_e, elem=elem, q_mask=q, material=np.asarray(material, dtype=np.int64),
+                     elem_type=elem_type.upper(), surf=surf)
+    if reorder:
+        mesh = reorder_nodes(mesh, xyz_order(mesh.coord))
+    return mesh

@@ -1,0 +1,67 @@
+"""Command-line runner: ``python -m slope_stability_petsc4py_amd.run --case homo2d --precision fp64``.
+
+Runs one SSR (or LL) continuation on the device and prints one JSON line with the factor of
+safety (or limit load), the histories, wall time and iteration counts.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import cases
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--case", default="homo2d", choices=["homo2d", "hetero3d_L1", "slope3d_homo", "slope3d_hetero"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--pc", default="pmg", choices=["pmg", "cheb"])
+    ap.add_argument("--analysis", default="ssr", choices=["ssr", "ll"])
+    ap.add_argument("--h", type=float, default=None)
+    ap.add_argument("--nz", type=int, default=None)
+    ap.add_argument("--step-max", type=int, default=None)
+    ap.add_argument("--verbose", action="store_true")
+    a = ap.parse_args(argv)
+    t0 = time.perf_counter()
+    if a.case == "homo2d":
+        case = cases.homo2d(h=a.h or 1.0)
+    elif a.case == "hetero3d_L1":
+        case = cases.hetero3d_L1()
+    else:
+        case = cases.slope3d(h=a.h or 2.0, nz=a.nz or 10, hetero=a.case.endswith("hetero"))
+    if a.step_max:
+        case.params.step_max = a.step_max
+    s = cases.make_solver(case, precision=a.precision, pc=a.pc)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+
+    def prog(ev):
+        if a.verbose:
+            print(json.dumps({k: (float(v) if isinstance(v, (np.floating, float)) else v) for k, v in ev.items()}),
+                  file=sys.stderr, flush=True)
+
+    if a.analysis == "ssr":
+        res = cases.run_ssr(s, progress=prog)
+        out = {"case": case.name, "precision": a.precision, "pc": a.pc, "fos": res.fos,
+               "lambda_hist": res.lambda_hist.tolist(), "omega_hist": res.omega_hist.tolist(),
+               "Umax_hist": res.Umax_hist.tolist(), "stop_reason": res.stop_reason,
+               "wall_time_s": res.wall_time, "setup_s": t_setup, "newton_iterations": res.newton_iterations,
+               "linear_iterations": res.linear_iterations, "attempts": res.attempts,
+               "n_dof": int(s.op.pattern.n_rows), "n_elem": int(case.mesh.n_elem),
+               "timings": s.op.get_total_time()}
+    else:
+        res = cases.run_ll(s, progress=prog)
+        out = {"case": case.name, "precision": a.precision, "limit_load": res.limit_load,
+               "t_hist": res.t_hist.tolist(), "wall_time_s": res.wall_time, "stop_reason": res.stop_reason,
+               "newton_iterations": res.newton_iterations, "linear_iterations": res.linear_iterations}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -282,3 +282,25 @@ def test_operator_api_2d(gpu):
     Ko = np.asarray(Kg[rows, pat.indices]).ravel()
     Ko[pat.constrained_diag_positions] = 1.0
     assert _rel(K.values.cpu().numpy(), Ko) < 1e-10
+
+
+def test_p4_element_matrices_mfma_vs_oracle(gpu):
+    """P4 element tangents on the FP64 matrix cores vs the oracle's dense B^T D B, then the row
+    gather into the pattern vs the oracle rows kernel."""
+    print("case: P4 MFMA element matrices", flush=True)
+    m = mesh.slope_3d(h=5.0, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=1, elem_type="P4")
+    geom = fem.geometry(m.coord, m.elem, "P4")
+    pat = pattern.build_owned_pattern(m.elem, m.q_mask)
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    rng = np.random.default_rng(31)
+    A = rng.normal(size=(geom.n_int, 6, 6))
+    DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
+    DSd = torch.from_numpy(DS).to(dev)
+    ke = dp.element_matrices_p4(DSd).cpu().numpy()
+    keo = oracle.element_matrices_3d(geom.dphi, DS, geom.weight, geom.n_elem, geom.n_q)
+    assert _rel(ke, keo) < 1e-12
+    K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    dp.tangent(DSd, K)
+    Ko = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+    assert _rel(K.cpu().numpy(), Ko) < 1e-12

@@ -1,0 +1,72 @@
+"""Device MLP (bf16 MFMA GEMMs, tanh, Adam) against the CPU oracle; training reduces the loss."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp as O
+from slope_stability_petsc4py_amd.learn import NodeMLP
+
+pytestmark = pytest.mark.gpu
+
+
+def _load(net, W, b):
+    net.load_state_dict({"W": [w.astype(np.float32) for w in W], "b": [x.astype(np.float32) for x in b]})
+
+
+def test_forward_backward_vs_oracle(gpu):
+    print("case: MLP fwd/bwd vs oracle", flush=True)
+    rng = np.random.default_rng(0)
+    net = NodeMLP(n_in=16, hidden=128, layers=4, n_out=3, seed=1, zero_last=False)
+    W = [w.cpu().numpy().astype(np.float64) for w in net.W]
+    b = [rng.normal(size=x.numel()) * 0.1 for x in net.b]
+    _load(net, W, b)
+    W = [w.cpu().numpy().astype(np.float64) for w in net.W]
+    b = [x.cpu().numpy().astype(np.float64) for x in net.b]
+    M = 1000 + 37  # not a multiple of 16
+    X = rng.normal(size=(M, 16)).astype(np.float32)
+    T = rng.normal(size=(M, 3)).astype(np.float32)
+    out = net.forward(torch.from_numpy(X).to(gpu))
+    acts = O.forward(W, b, X)
+    np.testing.assert_allclose(out.cpu().numpy(), acts[-1], rtol=0, atol=2e-2 * np.abs(acts[-1]).max())
+    loss, g = net.mse(out, torch.from_numpy(T).to(gpu))
+    lo, go = O.mse(acts[-1], T)
+    assert abs(loss - lo) < 2e-2 * lo
+    net.backward(g)
+    gW, gb = O.backward(W, acts, go)
+    for li in range(len(W)):
+        gd = net.gW[li].cpu().numpy()
+        assert np.abs(gd - gW[li]).max() < 3e-2 * np.abs(gW[li]).max(), li
+        assert np.abs(net.gb[li].cpu().numpy() - gb[li]).max() < 3e-2 * np.abs(gb[li]).max(), li
+
+
+def test_training_reduces_loss(gpu):
+    print("case: MLP training", flush=True)
+    rng = np.random.default_rng(3)
+    M = 4096
+    X = rng.normal(size=(M, 16)).astype(np.float32)
+    T = np.stack([np.sin(X[:, 0]) * X[:, 7], 0.5 * X[:, 8] * X[:, 9], np.tanh(X[:, 13])], axis=1).astype(np.float32)
+    net = NodeMLP(seed=2)
+    Xd, Td = torch.from_numpy(X).to(gpu), torch.from_numpy(T).to(gpu)
+    losses = []
+    for _ in range(150):
+        out = net.forward(Xd)
+        loss, g = net.mse(out, Td)
+        net.backward(g)
+        net.adam(3e-3)
+        losses.append(loss)
+    assert losses[-1] < 0.2 * losses[0], (losses[0], losses[-1])
+
+
+def test_adam_matches_oracle(gpu):
+    net = NodeMLP(n_in=16, hidden=16, layers=2, n_out=3, seed=4, zero_last=False)
+    w0 = net.W[0].cpu().numpy().astype(np.float64)
+    g = np.random.default_rng(5).normal(size=w0.shape).astype(np.float32)
+    net.gW[0].copy_(torch.from_numpy(g))
+    for li in range(1, len(net.W)):
+        net.gW[li].zero_()
+    for li in range(len(net.b)):
+        net.gb[li].zero_()
+    net.adam(1e-2)
+    w1, _, _ = O.adam(w0, g.astype(np.float64), np.zeros_like(w0), np.zeros_like(w0), 1, lr=1e-2)
+    np.testing.assert_allclose(net.W[0].cpu().numpy(), w1, rtol=1e-5, atol=1e-6)
