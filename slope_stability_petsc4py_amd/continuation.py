@@ -208,7 +208,8 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         if step >= step_max:
             stop_reason = "step_max"
             break
-    torch.cuda.synchronize()
+    if f.is_cuda:
+        torch.cuda.synchronize()
     return SSRResult(np.asarray(lambda_hist), np.asarray(omega_hist), np.asarray(umax_hist), U, stop_reason,
                      perf_counter() - t0, stats.newton_iterations,
                      solver.iteration_collector.get_total_iterations() - lin0, attempts, step_U, step_info)
@@ -295,7 +296,8 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
         if len(t_hist) >= step_max:
             stop_reason = "step_max"
             break
-    torch.cuda.synchronize()
+    if f.is_cuda:
+        torch.cuda.synchronize()
     return LLResult(np.asarray(t_hist), np.asarray(omega_hist), np.asarray(umax_hist), U, stop_reason,
                     perf_counter() - t0, stats.newton_iterations,
                     solver.iteration_collector.get_total_iterations() - lin0)

@@ -1,0 +1,184 @@
+"""Newton / continuation control flow on the CPU with fake builders and solvers (the reference tests
+its drivers the same way: tests/test_newton_cleanup.py, tests/test_ssr_indirect_init.py,
+tests/test_damping.py).  The device dot kernel is replaced by a host dot for these tests only.
+
+Model problem, per dof: F_lambda(U) = (k c / lambda) tanh(U / u0), load f.  Its limit strength
+factor is lambda* = k c / f exactly (tanh -> 1), so the continuation must approach it from below.
+"""
+
+from types import SimpleNamespace
+
+import numpy as np
+import pytest
+import torch
+
+from slope_stability_petsc4py_amd import continuation, nonlinear
+
+
+class _Scr:
+    def __init__(self, dev=None):
+        pass
+
+
+def _dot(x, y, scr=None):
+    return float(torch.dot(x, y).item())
+
+
+@pytest.fixture(autouse=True)
+def host_dot(monkeypatch):
+    monkeypatch.setattr(nonlinear, "dot", _dot)
+    monkeypatch.setattr(nonlinear, "_Scratch", _Scr)
+    monkeypatch.setattr(continuation, "dot", _dot)
+    monkeypatch.setattr(continuation, "_Scratch", _Scr)
+
+
+class TanhBuilder:
+    def __init__(self, n=6, k=2.0, c=3.0, u0=0.1):
+        self.k, self.c, self.u0 = k, c, u0
+        self.lam = None
+        self.U = None
+        self.calls = {"tangent": 0, "stress": 0}
+
+    def reduction(self, lam):
+        if lam <= 0:
+            raise ValueError("Reduction parameter lambda must be positive")
+        self.lam = lam
+
+    def _F(self, U):
+        return self.k * self.c / self.lam * torch.tanh(U / self.u0)
+
+    def constitutive_problem_stress_tangent(self, U):
+        self.U = U.clone()
+        self.calls["tangent"] += 1
+
+    def constitutive_problem_stress(self, U):
+        self.U = U.clone()
+        self.calls["stress"] += 1
+
+    def build_F_vec(self):
+        return self._F(self.U)
+
+    def build_F_all_vec(self, lam, U):
+        self.reduction(lam)
+        self.constitutive_problem_stress(U)
+        return self._F(U)
+
+    def build_F_reduced_vec(self, U):
+        self.constitutive_problem_stress(U)
+        return self._F(U)
+
+    def build_K_regularized(self, r):
+        d = self.k * self.c / self.lam / self.u0 * (1 - torch.tanh(self.U / self.u0) ** 2)
+        el = self.k * self.c / self.lam / self.u0
+        return SimpleNamespace(diag=(1 - r) * d + r * el)
+
+
+class DiagSolver:
+    def __init__(self):
+        self.basis = []
+        self.iteration_collector = SimpleNamespace(get_total_iterations=lambda: 0)
+
+    def setup_preconditioner(self, A, **kw):
+        self.A = A
+
+    def A_orthogonalize(self, A):
+        pass
+
+    def solve(self, A, b, **kw):
+        return b / A.diag
+
+    def expand_deflation_basis(self, v):
+        self.basis.append(v.clone())
+
+    def get_deflation_basis_snapshot(self):
+        return list(self.basis)
+
+    def restore_deflation_basis(self, snap):
+        self.basis = list(snap)
+
+
+def test_plain_newton_converges_and_expands_basis():
+    b = TanhBuilder()
+    s = DiagSolver()
+    f = torch.full((6,), 1.0, dtype=torch.float64)
+    b.reduction(1.0)
+    U, flag, it = nonlinear.newton(torch.zeros(6, dtype=torch.float64), 1e-10, 50, 10, 1e-4, f, b, s,
+                                   free_mask=torch.ones(6, dtype=torch.float64))
+    assert flag == 0
+    np.testing.assert_allclose(b._F(U).numpy(), 1.0, rtol=1e-9)
+    assert len(s.basis) >= 1
+
+
+def test_newton_fails_beyond_limit_and_flags():
+    b = TanhBuilder()
+    s = DiagSolver()
+    f = torch.full((6,), 1.0, dtype=torch.float64)
+    b.reduction(10.0)  # k c / lambda = 0.6 < f: no solution
+    U, flag, it = nonlinear.newton(torch.zeros(6, dtype=torch.float64), 1e-10, 30, 10, 1e-4, f, b, s,
+                                   free_mask=torch.ones(6, dtype=torch.float64))
+    assert flag == 1
+
+
+def test_damping_alg5_skips_nonpositive_lambda():
+    b = TanhBuilder()
+    f = torch.full((4,), 1.0, dtype=torch.float64)
+    U = torch.full((4,), 0.05, dtype=torch.float64)
+    b.reduction(1.0)
+    crit = float(torch.linalg.vector_norm(b._F(U) - f))
+    # a step that would drive lambda negative at alpha = 1 must halve first
+    a = nonlinear.damping_alg5(10, U, 1.0, torch.zeros(4, dtype=torch.float64), -1.5, f, crit, b, None)
+    assert 0.0 < a <= 0.5
+
+
+def test_ssr_continuation_approaches_limit_factor():
+    k, c, fval = 2.0, 3.0, 1.0
+    b = TanhBuilder(k=k, c=c)
+    s = DiagSolver()
+    n = 6
+    f = torch.full((n,), fval, dtype=torch.float64)
+    res = continuation.ssr_indirect_continuation(
+        1.0, 0.5, 1e-5, 1e-4, 60, 1e9, 50, 10, 1e-9, 1e-4, f, b, s,
+        free_mask=torch.ones(n, dtype=torch.float64), dim=3)
+    lam_star = k * c / fval
+    assert res.lambda_hist[-1] < lam_star
+    assert res.lambda_hist[-1] > lam_star - 0.05
+    assert np.all(np.diff(res.lambda_hist) > 0)
+    assert np.all(np.diff(res.omega_hist) > 0)
+    assert res.stop_reason in {"d_lambda_diff_scaled_min", "step_max"}
+
+
+def test_learned_predictor_gate_rejects_bad_guess():
+    b = TanhBuilder()
+    s = DiagSolver()
+    n = 6
+    f = torch.full((n,), 1.0, dtype=torch.float64)
+    calls = {"n": 0}
+
+    def bad_predictor(ctx):
+        calls["n"] += 1
+        # far off the branch (the symmetric branch has all dofs equal): first residual worse than the secant's
+        return torch.tensor([5.0, -3.0, 5.0, -3.0, 5.0, -3.0], dtype=torch.float64)
+
+    res = continuation.ssr_indirect_continuation(
+        1.0, 0.5, 1e-5, 1e-3, 8, 1e9, 50, 10, 1e-9, 1e-4, f, b, s,
+        free_mask=torch.ones(n, dtype=torch.float64), dim=3, predictor=bad_predictor)
+    assert calls["n"] > 0
+    kinds = {r["predictor"] for r in res.step_info}
+    assert "learned" not in kinds and "learned_gated" in kinds
+
+
+def test_learned_predictor_exact_guess_is_used():
+    k, c = 2.0, 3.0
+    b = TanhBuilder(k=k, c=c)
+    s = DiagSolver()
+    n = 6
+    f = torch.full((n,), 1.0, dtype=torch.float64)
+
+    def exact(ctx):
+        # the branch point at omega_target: U_i = omega/n, lambda = k c tanh(U/u0) / f
+        return torch.full((n,), ctx["omega_target"] / n, dtype=torch.float64)
+
+    res = continuation.ssr_indirect_continuation(
+        1.0, 0.5, 1e-5, 1e-3, 8, 1e9, 50, 10, 1e-9, 1e-4, f, b, s,
+        free_mask=torch.ones(n, dtype=torch.float64), dim=3, predictor=exact)
+    assert any(r["predictor"] == "learned" for r in res.step_info)

@@ -90,6 +90,43 @@ def test_tet10_elevation_first_encounter_order():
     np.testing.assert_allclose(c10[:, 11], 0.5 * (coord[:, 2] + coord[:, 4]))
 
 
+@pytest.mark.parametrize("et,nodes,constrained,free", [("P2", 27605, (395, 722, 1336), 80362),
+                                                       ("P4", 208549, (1472, 2783, 5070), 616322)])
+def test_hetero_L1_known_answers(et, nodes, constrained, free):
+    """Known answers of the reference's loader tests (tests/test_3d_hetero_ssr_gmsh_loader.py:40-93)
+    and run sizes (80,362 P2 unknowns of the 3D hetero SSR capture; 616,322 P4 unknowns)."""
+    from pathlib import Path
+    d = np.load(Path(__file__).resolve().parent / "golden" / "ssr_hetero_L1_tet4.npz")
+    assert d["coord"].shape == (3, 3845) and d["tet4"].shape == (4, 18419) and d["tri3"].shape == (3, 6325)
+    m = mesh.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
+                                  d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,
+                                  {"x": [1, 2], "y": [5], "z": [3, 4]}, et)
+    assert m.coord.shape[1] == nodes
+    assert tuple(int(v) for v in (~m.q_mask).sum(axis=1)) == constrained
+    assert int(m.q_mask.sum()) == free
+    if et == "P4":
+        # every local node sits at its barycentric lattice position
+        tup = fem.tetra_node_tuples(4)
+        e = 77
+        v = m.coord[:, m.elem[:4, e]]
+        for k, t in enumerate(tup):
+            np.testing.assert_allclose(m.coord[:, m.elem[k, e]], sum(t[i] / 4.0 * v[:, i] for i in range(4)),
+                                       atol=1e-9)
+
+
+def test_gmsh_reader_binary_and_ascii_agree(tmp_path):
+    coord = np.array([[0, 1, 0, 0], [0, 0, 1, 0], [0, 0, 0, 1.0]])
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$Nodes", "4"]
+    lines += [f"{i + 1} {coord[0, i]} {coord[1, i]} {coord[2, i]}" for i in range(4)]
+    lines += ["$EndNodes", "$Elements", "2", "1 2 2 3 1 1 2 3", "2 4 2 2 1 1 2 3 4", "$EndElements"]
+    p = tmp_path / "a.msh"
+    p.write_text("\n".join(lines) + "\n")
+    c, t4, tp, tri, trp = mesh.read_gmsh22(p)
+    np.testing.assert_allclose(c, coord)
+    np.testing.assert_array_equal(t4[:, 0], [0, 1, 2, 3])
+    assert tp.tolist() == [2] and trp.tolist() == [3] and tri[:, 0].tolist() == [0, 1, 2]
+
+
 def test_xyz_order_is_x_major():
     m = mesh.slope_3d(h=5.0, nz=2, x1=10, x2=10, x3=10, y1=10, y2=10, z_len=10)
     c = m.coord
