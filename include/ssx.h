@@ -131,6 +131,12 @@ int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double *ke,
                             void *stream);
 
+/* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p) by 3x3 node
+ * blocks on the vector ALUs (one lane per node pair i <= j, mirrored).  Same reference stage as
+ * ssx_element_matrices_p4 (cython/assemble_tangent_values_3d.c:45-94). */
+int ssx_element_matrices_blocks(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem,
+                                int n_q, double *ke, void *stream);
+
 /* Row gather of element matrices into the fixed CSR pattern through the row-slot lists (slot order
  * = element order), with unit constrained diagonals, optional K_r blend and FP32 copy.  Together
  * with ssx_element_matrices_p4 replaces assemble_tangent_values_3d_rows_c for P4

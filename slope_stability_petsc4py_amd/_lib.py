@@ -51,6 +51,7 @@ _SIGS: dict[str, list] = {
     "ssx_f32_to_bf16": [I64, P, P, P],
     "ssx_bf16_to_f32": [I64, P, P, P],
     "ssx_mse_f32": [I64, P, P, P, F32, P, P, P],
+    "ssx_element_matrices_blocks": [I32, P, P, P, I64, I32, P, P],
     "ssx_stream_sync": [P],
 }
 

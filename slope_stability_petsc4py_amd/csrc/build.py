@@ -25,6 +25,7 @@ SOURCES = [
     "sparse.hip",
     "p4_mfma.hip",
     "mlp.hip",
+    "elem_blocks.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

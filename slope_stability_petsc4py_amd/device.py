@@ -147,10 +147,20 @@ class DevicePattern:
                   stream_ptr())
         return self._ke
 
+    def element_matrices_blocks(self, DS: torch.Tensor) -> torch.Tensor:
+        """(n_elem, nld, nld) element tangents of a P1/P2 tetrahedral mesh (node-block kernel)."""
+        if getattr(self, "_ke", None) is None:
+            self._ke = torch.empty((self.n_elem, self.nld, self.nld), dtype=torch.float64, device=DS.device)
+        _lib.call("ssx_element_matrices_blocks", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
+                  self.n_q, ptr(self._ke), stream_ptr())
+        return self._ke
+
+    tangent_method = "rows"   # P1/P2 3D: "rows" (fused row kernel) or "elem" (element blocks + row gather)
+
     def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
                 elastic: torch.Tensor | None = None) -> None:
-        if self.dim == 3 and self.n_p == 35:
-            ke = self.element_matrices_p4(DS)
+        if self.dim == 3 and (self.n_p == 35 or self.tangent_method == "elem"):
+            ke = self.element_matrices_p4(DS) if self.n_p == 35 else self.element_matrices_blocks(DS)
             _lib.call("ssx_scatter_element_rows", ptr(ke), self.nld, ptr(self.row_slot_ptr), ptr(self.slot_elem),
                       ptr(self.slot_lrow), ptr(self.slot_pos), ptr(self.indptr), ptr(self.row_free), self.n_rows,
                       float(r), ptr(elastic), ptr(out), ptr(out32), self.max_block_nnz, stream_ptr())

@@ -1,0 +1,97 @@
+"""Learned-predictor experiment: seeded 3D homogeneous slopes (height, face angle, c, phi drawn per
+seed), their own SSR histories as training data, evaluation on held-out seeds against the plain
+secant continuation (Newton iterations, wall time, FoS difference).
+
+``python -m slope_stability_petsc4py_amd.learn_experiment --train 12 --test 4``
+prints one JSON line.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+import torch
+
+from . import cases, mesh
+from .learn import LearnedPredictor, NodeMLP, node_info, samples_from_history, train
+from .materials import MaterialSpec
+
+
+def slope_case(seed: int, h: float = 2.0, nz: int = 6) -> cases.Case:
+    rng = np.random.default_rng(seed)
+    height = float(rng.uniform(8.0, 15.0))
+    beta = float(rng.uniform(30.0, 60.0))
+    c0 = float(rng.uniform(5.0, 20.0))
+    phi = float(rng.uniform(25.0, 40.0))
+    run = height / np.tan(np.deg2rad(beta))
+    m = mesh.slope_3d(h=h, x1=10.0, x2=run, x3=10.0, y1=6.0, y2=height, z_len=12.0, nz=nz)
+    mat = MaterialSpec(c0=c0, phi=phi, psi=0.0, young=40000.0, poisson=0.3, gamma_sat=20.0, gamma_unsat=20.0)
+    p = cases.SSRParams(lambda_init=0.7, d_lambda_init=0.1, d_lambda_min=1e-4, d_lambda_diff_scaled_min=1e-3,
+                        step_max=40, omega_max_stop=1e12, it_newt_max=60)
+    c = cases.Case(f"slope_seed{seed}", m, [mat], p)
+    c.extra = {"height": height, "beta": beta, "c0": c0, "phi": phi}
+    return c
+
+
+def run_case(case, predictor=None, precision="fp64", store=False):
+    s = cases.make_solver(case, precision=precision)
+    pred = predictor(case) if callable(predictor) else None
+    res = cases.run_ssr(s, predictor=pred, store_step_u=store)
+    return s, res
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=12)
+    ap.add_argument("--test", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--h", type=float, default=2.0)
+    ap.add_argument("--nz", type=int, default=6)
+    a = ap.parse_args(argv)
+    log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    t0 = time.perf_counter()
+    samples = []
+    train_info = []
+    for seed in range(a.train):
+        case = slope_case(seed, a.h, a.nz)
+        s, res = run_case(case, store=True)
+        info = node_info(case.mesh, case.materials)
+        samples += samples_from_history(info, res.step_U, res.omega_hist, res.lambda_hist)
+        train_info.append({"seed": seed, "fos": res.fos, "newton": res.newton_iterations, "steps": len(res.lambda_hist)})
+        log("train run", train_info[-1])
+        del s, res
+        torch.cuda.empty_cache()
+    t_data = time.perf_counter() - t0
+    net = NodeMLP(seed=0)
+    t1 = time.perf_counter()
+    losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    evals = []
+    for seed in range(1000, 1000 + a.test):
+        case = slope_case(seed, a.h, a.nz)
+        _, base = run_case(case)
+        info = node_info(case.mesh, case.materials)
+        _, lrn = run_case(case, predictor=lambda c: LearnedPredictor(net, info))
+        used = sum(1 for r in lrn.step_info if r["predictor"] == "learned")
+        evals.append({"seed": seed, "fos_plain": base.fos, "fos_learned": lrn.fos,
+                      "fos_abs_diff": abs(base.fos - lrn.fos), "newton_plain": base.newton_iterations,
+                      "newton_learned": lrn.newton_iterations, "wall_plain": base.wall_time,
+                      "wall_learned": lrn.wall_time, "learned_steps_used": used,
+                      "attempts_plain": base.attempts, "attempts_learned": lrn.attempts, **case.extra})
+        log("eval", evals[-1])
+    out = {"train_seeds": a.train, "test_seeds": a.test, "samples": len(samples), "epochs": a.epochs,
+           "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
+           "data_gen_s": t_data, "train_s": t_train, "train_runs": train_info, "eval": evals,
+           "newton_ratio": float(np.sum([e["newton_learned"] for e in evals]) /
+                                 max(1, np.sum([e["newton_plain"] for e in evals]))),
+           "max_fos_diff": float(max(e["fos_abs_diff"] for e in evals)) if evals else None}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
