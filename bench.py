@@ -86,6 +86,7 @@ def run_gpu(args):
 
     m, op, E_host, br = build_workload(args.h, args.nz, args.unique, args.seed)
     pat = op.pattern
+    pat.tangent_method = args.tangent
     dev = op._E.device
     S, DS = op._S, op._DS
     K = op.pattern.new_csr(with32=True)
@@ -186,8 +187,8 @@ def cpu_baseline(op, E_host, ab, budget_s: float):
     geom = op.geom
     pat = op.pattern.host
     n_e_total = geom.n_elem
-    # sample: the first n_e elements' points (and the rows they touch are approximated by the full
-    # row lists restricted to those elements); time scales linearly in points, reported per byte
+    # the whole workload (same strains, geometry and row-slot metadata) with the host's OpenMP threads;
+    # repeated until the time budget is spent (at most 5 repetitions)
     n = geom.n_int
     cb = np.full(n, 4 * np.sqrt(3))
     sp = np.full(n, 1 / np.sqrt(3))
@@ -237,6 +238,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tangent", default="rows", choices=["rows", "elem"],
+                    help="P2 tangent path: fused row kernel or element blocks + row gather")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the boxes here have one GPU); "
@@ -276,7 +279,7 @@ def main():
                         "(FP64, FP32 shadow)",
             "n_elem": int(op.pattern.n_elem), "n_points": int(op.n_int), "n_dof": int(op.pattern.n_rows),
             "nnz": int(op.pattern.host.nnz), "branch_hist_el_smooth_left_right_apex": res["hist"],
-            "parallelism": "single GPU",
+            "parallelism": "single GPU", "tangent_path": args.tangent,
         },
         "roofline": {
             "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -22,7 +22,7 @@ from . import mesh as M
 from .constitutive import ConstitutiveOperator
 from .continuation import ll_indirect_continuation, ssr_indirect_continuation
 from .device import to_dev
-from .linear import ChebyshevJacobi, DeflatedFGMRES, PMultigrid
+from .linear import ChebyshevJacobi, DeflatedFGMRES, PMultigrid, PMultigridP4
 from .materials import MaterialSpec
 from .workloads import HETERO_LAYERS, HOMO
 
@@ -70,6 +70,7 @@ def homo2d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.0)
 
 
 def hetero3d_L1(elem_type: str = "P2") -> Case:
+    """elem_type "P4" gives the reference's P4 L1 problem (616,322 free dofs)."""
     d = np.load(DATA / "ssr_hetero_L1_tet4.npz", allow_pickle=False)
     m = M.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
                                d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,
@@ -102,7 +103,9 @@ def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg") -> Solv
     f_field = op.gravity_load(case.mesh)
     f = to_dev(f_field.T.reshape(-1).copy())
     free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
-    if pc == "pmg":
+    if pc == "pmg" and case.mesh.elem_type == "P4":
+        prec = PMultigridP4(case.mesh, op, use32=use32)
+    elif pc == "pmg":
         prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32)
     else:
         prec = ChebyshevJacobi(use32=use32)

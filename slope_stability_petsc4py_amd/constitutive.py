@@ -303,6 +303,7 @@ class ConstitutiveOperator:
         if self._owned_regularized_mat is None:
             self._owned_regularized_mat = self.pattern.new_csr(with32=self.fp32_shadow)
         K = self._owned_regularized_mat
+        self.last_regularization = float(r)
         t0 = perf_counter()
         self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), elastic=self.elastic_values)
         self.time_build_tangent_local.append(perf_counter() - t0)

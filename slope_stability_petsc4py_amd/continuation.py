@@ -232,68 +232,76 @@ class LLResult:
 
 
 def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_stop, it_newt_max, it_damp_max, tol,
-                             r_min, f, builder, solver, *, free_mask, dim, progress=None) -> LLResult:
-    """Limit-load continuation in omega = f^T U (continuation/limit_load.py:66-374): an elastic
-    solve gives U_el; omega starts at d_omega_ini_scale * f^T U_el; steps grow after easy steps and
-    halve on failure; stops at omega_max_stop, step_max, 5 reductions, or d_t < d_t_min."""
+                             r_min, f, builder, solver, *, free_mask, dim, lambda_ell: float = 1.0,
+                             progress=None) -> LLResult:
+    """Limit-load continuation in omega = f^T U (continuation/limit_load.py:66-374 with the driver's
+    elastic start, cli/run_3D_hetero_SSR_capture.py:1224-1305): one elastic solve K_el U_el = f
+    (strength reduced by lambda_ell), omega_el = f^T U_el, start from U = scale*U_el, t = 0, first
+    omega step scale*omega_el; secant predictor in omega; d_omega halves on failure (5 in a row
+    stop), doubles after an easy step (< 20 Newton iterations) whose t increment shrank by > 10%;
+    stop at d_t < d_t_min, omega_max_stop or step_max.  The limit load factor is t_hist[-1]."""
     t0 = perf_counter()
     stats = NewtonStats()
     scr = _Scratch(f.device)
     lin0 = solver.iteration_collector.get_total_iterations()
-    builder.reduction(1.0)
-    U_el, flag, _ = newton(torch.zeros_like(f), tol, it_newt_max, it_damp_max, r_min, f, builder, solver,
-                           free_mask=free_mask, stats=stats)
+    builder.reduction(float(lambda_ell))
+    K_el = builder.elastic_matrix()
+    solver.setup_preconditioner(K_el)
+    solver.A_orthogonalize(K_el)
+    U_el = solver.solve(K_el, f.clone()).mul_(free_mask)
+    solver.expand_deflation_basis(U_el)
     omega_el = dot(f, U_el, scr)
-    d_omega = d_omega_ini_scale * omega_el
-    U_old = torch.zeros_like(f)
-    U = U_el * d_omega_ini_scale
-    # first point: F(U) = t f at omega = d_omega
-    U, t, flag, _ = newton_ind_ll(U, d_omega_ini_scale, d_omega, it_newt_max, it_damp_max, tol, r_min, f, builder,
-                                  solver, free_mask=free_mask, stats=stats)
-    t_hist = [0.0, t]
-    omega_hist = [0.0, d_omega]
-    umax_hist = [0.0, _umax(U, dim)]
-    omega = d_omega
-    omega_old = 0.0
+    d_omega = omega_el * float(d_omega_ini_scale)
+    U = U_el * float(d_omega_ini_scale)
+    U_old = U.clone()
+    omega = omega_old = 0.0
+    t = 0.0
+    t_hist, omega_hist, umax_hist = [0.0], [0.0], [_umax(U, dim)]
+    step = 1
     n_omega = 0
     stop_reason = "completed"
     while True:
-        omega_it = min(omega + d_omega, omega_max_stop)
-        d_omega = omega_it - omega
-        a = 0.0 if omega == omega_old else d_omega / (omega - omega_old)
-        U_ini = U + a * (U - U_old)
-        t_ini = t + a * (t - t_hist[-2])
+        omega_it = omega + d_omega
+        if omega_it > omega_max_stop:
+            omega_it = omega_max_stop
+            d_omega = omega_it - omega
+        if step > 1:
+            den = omega - omega_old
+            U_ini = U if den == 0.0 else U + (d_omega / den) * (U - U_old)
+        else:
+            U_ini = U
         basis_snap = solver.get_deflation_basis_snapshot()
-        U_it, t_it, flag, it = newton_ind_ll(U_ini, t_ini, omega_it, it_newt_max, it_damp_max, tol, r_min, f, builder,
-                                             solver, free_mask=free_mask, stats=stats)
+        U_it, t_it, flag, it_newt = newton_ind_ll(U_ini, t, omega_it, it_newt_max, it_damp_max, tol, r_min, f,
+                                                  builder, solver, free_mask=free_mask, stats=stats)
         solver.restore_deflation_basis(basis_snap)
         if progress is not None:
-            progress({"omega_target": omega_it, "flag": flag, "newton_it": it, "t": t_it})
-        if flag != 0:
+            progress({"omega_target": omega_it, "flag": flag, "newton_it": it_newt, "t": t_it})
+        if flag == 1:
             d_omega /= 2.0
             n_omega += 1
         else:
-            d_t = t_it - t
+            step += 1
             U_old, U = U, U_it
             solver.expand_deflation_basis(U)
             omega_old, omega = omega, omega_it
+            d_t = t_it - t
             t = t_it
+            n_omega = 0
             t_hist.append(t)
             omega_hist.append(omega)
             umax_hist.append(_umax(U, dim))
-            n_omega = 0
-            if len(t_hist) >= 3 and (t_hist[-1] - t_hist[-2]) < 0.9 * (t_hist[-2] - t_hist[-3]):
-                d_omega *= 2.0
-            if abs(d_t) < d_t_min:
+            if d_t < d_t_min:
                 stop_reason = "d_t_min"
                 break
-            if omega >= omega_max_stop:
-                stop_reason = "omega_max_stop"
-                break
+            if it_newt < 20 and step > 2 and (t_hist[-1] - t_hist[-2]) < 0.9 * (t_hist[-2] - t_hist[-3]):
+                d_omega *= 2.0
         if n_omega >= 5:
             stop_reason = "omega_reduction_limit"
             break
-        if len(t_hist) >= step_max:
+        if omega >= omega_max_stop:
+            stop_reason = "omega_max"
+            break
+        if step >= step_max:
             stop_reason = "step_max"
             break
     if f.is_cuda:

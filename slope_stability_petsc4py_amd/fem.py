@@ -240,14 +240,15 @@ class Geometry:
         return self.n_elem * self.n_q
 
 
-def geometry(coord: np.ndarray, elem: np.ndarray, elem_type: str) -> Geometry:
-    """dphi and weights for every (element, point); coord (dim, n_nodes), elem (n_p, n_elem)."""
+def geometry(coord: np.ndarray, elem: np.ndarray, elem_type: str, quad: tuple | None = None) -> Geometry:
+    """dphi and weights for every (element, point); coord (dim, n_nodes), elem (n_p, n_elem).
+    ``quad`` = (xi, wf) overrides the element's own rule (a coarser basis at a finer level's points)."""
     coord = np.asarray(coord, dtype=np.float64)
     elem = np.asarray(elem, dtype=np.int64)
     dim = coord.shape[0]
     n_p, n_e = elem.shape
     if dim == 3:
-        xi, wf = quadrature_3d(elem_type)
+        xi, wf = quadrature_3d(elem_type) if quad is None else quad
         hatp, h1, h2, h3 = basis_3d(elem_type, xi)
         n_q = xi.shape[1]
         cx, cy, cz = coord[0, elem], coord[1, elem], coord[2, elem]  # (n_p, n_e)

@@ -235,6 +235,144 @@ def p2_to_p1_prolongation(elem: np.ndarray, q_mask: np.ndarray):
     return rows_ptr, np.asarray(idx, dtype=np.int64), np.asarray(w, dtype=np.float64), dim * n_c_nodes, c_free
 
 
+_P4_TO_P2_LOCAL = (0, 1, 2, 3, 5, 8, 11, 14, 17, 20)  # P4 local nodes that are the P2 nodes
+
+
+def p4_to_p2_level(coord, elem35, q_mask):
+    """The P2 mesh inside a P4 mesh and the P4 <- P2 interpolation (P2 basis at the P4 nodes,
+    linear/pmg.py:365-436 for the same construction between P2 and P1).
+
+    Returns (coord2, elem2, q_mask2, P rows (ptr, idx, w) over fine dofs, n_c dofs)."""
+    from .fem import basis_3d, tetra_node_tuples
+    elem35 = np.asarray(elem35, dtype=np.int64)
+    dim, n_f_nodes = q_mask.shape
+    sub = elem35[list(_P4_TO_P2_LOCAL)]
+    nodes2 = np.unique(sub.reshape(-1))
+    num2 = np.full(n_f_nodes, -1, dtype=np.int64)
+    num2[nodes2] = np.arange(nodes2.size)
+    elem2 = num2[sub]
+    coord2 = coord[:, nodes2]
+    q2 = q_mask[:, nodes2]
+    tup = np.array(tetra_node_tuples(4), dtype=np.float64)[:, 1:].T / 4.0   # (3, 35) reference coords
+    phi = basis_3d("P2", tup)[0]                                             # (10, 35)
+    free_f = q_mask.T.reshape(-1)
+    free_c = q2.T.reshape(-1)
+    seen = np.zeros(n_f_nodes, dtype=bool)
+    rows = {}
+    for k in range(35):
+        nodes_k = elem35[k]
+        new = ~seen[nodes_k]
+        seen[nodes_k] = True
+        idx = np.flatnonzero(new)
+        for e in idx:
+            wts = phi[:, k]
+            nz = np.flatnonzero(np.abs(wts) > 1e-14)
+            rows[int(nodes_k[e])] = (elem2[nz, e], wts[nz])
+    p_ptr = np.zeros(dim * n_f_nodes + 1, dtype=np.int64)
+    pi, pw = [], []
+    for node in range(n_f_nodes):
+        cn, wts = rows[node]
+        for c in range(dim):
+            f = dim * node + c
+            if free_f[f]:
+                for J, wt in zip(cn, wts):
+                    if free_c[dim * J + c]:
+                        pi.append(dim * int(J) + c)
+                        pw.append(float(wt))
+            p_ptr[f + 1] = len(pi)
+    return coord2, elem2, q2, (p_ptr, np.asarray(pi, dtype=np.int64), np.asarray(pw)), dim * nodes2.size
+
+
+class _Transfer:
+    """Prolongation P (fine x coarse) and restriction P^T on the device, FP64 and FP32 values."""
+
+    def __init__(self, p_ptr, p_idx, p_w, n_f, n_c, dev):
+        import scipy.sparse as sp
+        P = sp.csr_matrix((p_w, p_idx, p_ptr), shape=(n_f, n_c))
+        PT = P.T.tocsr()
+        mk = lambda M: (torch.from_numpy(M.indptr.astype(np.int32)).to(dev),  # noqa: E731
+                        torch.from_numpy(M.indices.astype(np.int32)).to(dev),
+                        torch.from_numpy(M.data.astype(np.float64)).to(dev))
+        self.P = DeviceCSR(n_f, *mk(P), None, spmv_lanes(P.indptr))
+        self.P.values32 = self.P.values.to(torch.float32)
+        self.PT = DeviceCSR(n_c, *mk(PT), None, spmv_lanes(PT.indptr))
+        self.PT.values32 = self.PT.values.to(torch.float32)
+        self.p_ptr = torch.from_numpy(p_ptr.astype(np.int32)).to(dev)
+        self.p_idx = torch.from_numpy(p_idx.astype(np.int32)).to(dev)
+        self.p_w = torch.from_numpy(np.asarray(p_w, dtype=np.float64)).to(dev)
+
+
+class PMultigridP4:
+    """Three-level V-cycle P4 -> P2 -> P1 for order-4 tetrahedra.
+
+    The P2 level operator is rediscretised from the current tangent: the P2 basis integrated at the
+    P4 quadrature points with the same DS and regularisation (K_r at the P2 level); the P1 level is
+    the dense Galerkin product of the P2 level, factorised by Cholesky (as `PMultigrid`).  Chebyshev
+    (Jacobi) smoothing on the P4 and P2 levels, FP32 or FP64."""
+
+    def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1):
+        from .device import DevicePattern
+        from .fem import geometry, quadrature_3d
+        from .pattern import build_owned_pattern
+        self.op = op
+        self.use32 = use32
+        self.degree = degree
+        self.lo_frac = lo_frac
+        coord2, elem2, q2, prows, n2 = p4_to_p2_level(mesh.coord, mesh.elem, mesh.q_mask)
+        geom2 = geometry(coord2, elem2, "P2", quad=quadrature_3d("P4"))
+        pat2 = build_owned_pattern(elem2, q2)
+        self.pat2 = DevicePattern(pat2, geom2)
+        dev = self.pat2.indptr.device
+        self.t42 = _Transfer(*prows, mesh.q_mask.size, n2, dev)
+        # elastic values of the P2 level (same material points)
+        from .constitutive import constitutive_problem_3d
+        n = geom2.n_int
+        zeros = torch.zeros((n, 6), dtype=torch.float64, device=dev)
+        huge = torch.full((n,), 1e300, dtype=torch.float64, device=dev)
+        _S, DSel = constitutive_problem_3d(zeros, huge, torch.zeros(n, dtype=torch.float64, device=dev), op.shear,
+                                          op.lame, return_tangent=True)
+        self.el2 = torch.empty(pat2.nnz, dtype=torch.float64, device=dev)
+        self.pat2.tangent(DSel, self.el2)
+        self.A2 = self.pat2.new_csr(with32=True)
+        self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac)
+        self.level = None
+        self.r = 1e-4
+
+    def setup(self, A: DeviceCSR, mesh_info=None):
+        if self.level is None or self.level.A is not A:
+            self.level = _Level(A, self.use32)
+        self.level.refresh()
+        r = float(getattr(self.op, "last_regularization", self.r))
+        self.pat2.tangent(self.op._DS, self.A2.values, self.A2.values32, r=r, elastic=self.el2)
+        self.coarse.setup(self.A2)
+        dt = torch.float32 if self.use32 else torch.float64
+        dev = A.indptr.device
+        self.b2 = torch.empty(self.A2.n_rows, dtype=dt, device=dev)
+        self.x2 = torch.empty(self.A2.n_rows, dtype=dt, device=dev)
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        Lv = self.level
+        n = Lv.n
+        if self.use32:
+            b = torch.empty(n, dtype=torch.float32, device=r.device)
+            _lib.call("ssx_cast_f64_f32", n, ptr(r), ptr(b), _st())
+        else:
+            b = r
+        x = Lv.x
+        Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=True)
+        res = torch.empty_like(x)
+        axpby(1.0, b, 0.0, res)
+        spmv(Lv.A, x, res, alpha=-1.0, beta=1.0, use32=self.use32)
+        spmv(self.t42.PT, res, self.b2, use32=self.use32)
+        self.coarse.cycle(self.b2, self.x2)
+        spmv(self.t42.P, self.x2, x, alpha=1.0, beta=1.0, use32=self.use32)
+        Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=False)
+        if self.use32:
+            _lib.call("ssx_cast_f32_f64", n, ptr(x), ptr(z), _st())
+        else:
+            z.copy_(x)
+
+
 class PMultigrid:
     """Two-level V-cycle: Chebyshev(Jacobi) pre/post smoothing on the P2 operator, Galerkin coarse
     operator on the P1 vertex dofs solved by a dense Cholesky factorisation."""
@@ -295,15 +433,9 @@ class PMultigrid:
         self.Lc = L
         del Ac
 
-    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+    def cycle(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        """One V-cycle on A x = b in the level precision (x overwritten)."""
         Lv = self.level
-        n = Lv.n
-        if self.use32:
-            b = torch.empty(n, dtype=torch.float32, device=r.device)
-            _lib.call("ssx_cast_f64_f32", n, ptr(r), ptr(b), _st())
-        else:
-            b = r
-        x = Lv.x
         Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=True)          # pre-smooth
         res = torch.empty_like(x)
         axpby(1.0, b, 0.0, res)
@@ -313,6 +445,17 @@ class PMultigrid:
         self.xc.copy_(ec.to(self.xc.dtype))
         spmv(self.P, self.xc, x, alpha=1.0, beta=1.0, use32=self.use32)       # prolong + correct
         Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=False)          # post-smooth
+
+    def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
+        Lv = self.level
+        n = Lv.n
+        if self.use32:
+            b = torch.empty(n, dtype=torch.float32, device=r.device)
+            _lib.call("ssx_cast_f64_f32", n, ptr(r), ptr(b), _st())
+        else:
+            b = r
+        x = Lv.x
+        self.cycle(b, x)
         if self.use32:
             _lib.call("ssx_cast_f32_f64", n, ptr(x), ptr(z), _st())
         else:

@@ -234,8 +234,10 @@ def newton_ind_ssr(U_ini, omega, lambda_it, it_newt_max, it_damp_max, tol, r_min
 
 def newton_ind_ll(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, builder, solver, *, free_mask,
                   stats: NewtonStats | None = None):
-    """Nested Newton for F(U) = t f with f^T U = omega (nonlinear/newton.py:1131-1487).
-    Returns (U, t, flag, it)."""
+    """Nested Newton for F(U) = t f with f^T U = omega (nonlinear/newton.py:1131-1487): residual
+    |t f - F(U)| / |f|; solves K dW = f and K dV = t f - F; d_t = -f^T dV / f^T dW; damping of the
+    plain-Newton kind on (F, 0) (the reference passes a zero load to `damping`); U is rescaled to
+    omega and t advances by the full d_t.  Returns (U, t, flag, it)."""
     scr = _Scratch(U_ini.device)
     U = U_ini.clone()
     norm_f = _norm(f, scr) or 1.0
@@ -244,6 +246,7 @@ def newton_ind_ll(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, 
     flag = 0
     t = float(t_ini)
     compute_diffs = True
+    zero = torch.zeros_like(f)
     while True:
         it += 1
         if compute_diffs:
@@ -252,8 +255,7 @@ def newton_ind_ll(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, 
             K_r = builder.build_K_regularized(r)
         else:
             F = builder.build_F_reduced_vec(U).clone()
-        crit = _norm(F - t * f, scr)
-        rel = crit / (abs(t) * norm_f if t != 0 else norm_f)
+        rel = _norm(t * f - F, scr) / norm_f
         if stats is not None:
             stats.residual_evals += 1
         if compute_diffs and rel < tol and it > 1:
@@ -265,22 +267,9 @@ def newton_ind_ll(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, 
         dW = solver.solve(K_r, f.clone()).mul_(free_mask)
         dV = solver.solve(K_r, t * f - F).mul_(free_mask)
         denom = dot(f, dW, scr)
-        d_t = 0.0 if abs(denom) < 1e-30 else (omega - dot(f, U, scr) - dot(f, dV, scr)) / denom
+        d_t = 0.0 if abs(denom) < 1e-30 else -dot(f, dV, scr) / denom
         dU = dV + d_t * dW
-        # line search on the residual norm at the trial load factor
-        alpha = 1.0
-        Ua = torch.empty_like(U)
-        ok = False
-        for _ in range(int(it_damp_max)):
-            torch.add(U, dU, alpha=alpha, out=Ua)
-            Fa = builder.build_F_reduced_vec(Ua)
-            ca = _norm(Fa - (t + alpha * d_t) * f, scr)
-            if ca < crit:
-                ok = True
-                break
-            alpha *= 0.5
-        if not ok:
-            alpha = 0.0
+        alpha = damping(it_damp_max, U, dU, F, zero, builder, scr)
         compute_diffs = True
         if alpha < 1e-1:
             if alpha == 0.0:
@@ -296,16 +285,14 @@ def newton_ind_ll(U_ini, t_ini, omega, it_newt_max, it_damp_max, tol, r_min, f, 
         if stats is not None:
             stats.newton_iterations += 1
         if alpha == 0.0 and r > 1.0:
-            if rel > 10.0 * tol:
-                flag = 1
+            flag = 1
             break
         U.add_(dU, alpha=alpha)
         d = dot(f, U, scr)
         if d != 0.0:
             U.mul_(omega / d)
-        t = t + alpha * d_t
+        t = t + d_t
         if not np.isfinite(rel) or it == it_newt_max:
-            if rel > 10.0 * tol:
-                flag = 1
+            flag = 1
             break
     return U, float(t), flag, it

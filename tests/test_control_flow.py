@@ -73,6 +73,10 @@ class TanhBuilder:
         return SimpleNamespace(diag=(1 - r) * d + r * el)
 
 
+    def elastic_matrix(self):
+        return SimpleNamespace(diag=torch.zeros(6, dtype=torch.float64) + self.k * self.c / self.lam / self.u0)
+
+
 class DiagSolver:
     def __init__(self):
         self.basis = []
@@ -145,6 +149,25 @@ def test_ssr_continuation_approaches_limit_factor():
     assert np.all(np.diff(res.lambda_hist) > 0)
     assert np.all(np.diff(res.omega_hist) > 0)
     assert res.stop_reason in {"d_lambda_diff_scaled_min", "step_max"}
+
+
+def test_ll_continuation_approaches_limit_load():
+    """F_i(U) = k_i c tanh(U_i/u0) = t f_i has the limit load t* = min_i k_i c / f_i
+    (continuation/limit_load.py); per-dof k keeps the omega constraint non-degenerate."""
+    c, fval = 3.0, 1.5
+    k = torch.tensor([2.0, 2.5, 3.0, 2.2, 4.0, 2.8], dtype=torch.float64)
+    b = TanhBuilder(k=k, c=c)
+    b.U = torch.zeros(6, dtype=torch.float64)
+    s = DiagSolver()
+    n = 6
+    f = torch.full((n,), fval, dtype=torch.float64)
+    res = continuation.ll_indirect_continuation(0.2, 1e-4, 80, 1e9, 50, 10, 1e-10, 1e-4, f, b, s,
+                                                free_mask=torch.ones(n, dtype=torch.float64), dim=3)
+    t_star = float(k.min()) * c / fval
+    assert res.t_hist[0] == 0.0
+    assert np.all(np.diff(res.t_hist) > 0)
+    assert t_star - 0.02 < res.limit_load < t_star
+    assert res.stop_reason in {"d_t_min", "step_max"}
 
 
 def test_learned_predictor_gate_rejects_bad_guess():

@@ -1,0 +1,75 @@
+"""Device linear solvers and the end-to-end SSR against the reference's committed results."""
+
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+import torch
+
+from slope_stability_petsc4py_amd import cases, mesh
+from slope_stability_petsc4py_amd.constitutive import ConstitutiveOperator
+from slope_stability_petsc4py_amd.linear import ChebyshevJacobi, DeflatedFGMRES, PMultigrid
+from slope_stability_petsc4py_amd.workloads import HOMO
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+@pytest.fixture(scope="module")
+def elastic3d():
+    m = mesh.slope_3d(h=2.5, nz=4, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0)
+    op = ConstitutiveOperator.from_mesh(m, [HOMO])
+    K = op.elastic_matrix()
+    f = op.gravity_load(m).T.reshape(-1).copy()
+    return m, op, K, f
+
+
+@pytest.mark.parametrize("use32", [False, True])
+@pytest.mark.parametrize("pc", ["pmg", "cheb"])
+def test_fgmres_solves_elastic_system(gpu, elastic3d, use32, pc):
+    print(f"case: fgmres pc={pc} fp32={use32}", flush=True)
+    m, op, K, f = elastic3d
+    prec = PMultigrid(m.elem, m.q_mask, use32=use32) if pc == "pmg" else ChebyshevJacobi(use32=use32)
+    solver = DeflatedFGMRES(prec, tolerance=1e-8, max_iterations=300)
+    solver.setup_preconditioner(K)
+    solver.A_orthogonalize(K)
+    b = torch.from_numpy(f).to(gpu)
+    x = solver.solve(K, b).cpu().numpy()
+    Ks = K.to_scipy()
+    x_ref = spla.spsolve(Ks.tocsc(), f)
+    assert np.linalg.norm(Ks @ x - f) <= 1.1e-8 * np.linalg.norm(f)
+    assert np.linalg.norm(x - x_ref) <= 1e-6 * np.linalg.norm(x_ref)
+    its = solver.iteration_collector.iterations[-1]
+    if pc == "pmg":
+        assert its < 60, its
+
+
+def test_deflation_basis_reuses_previous_solutions(gpu, elastic3d):
+    m, op, K, f = elastic3d
+    solver = DeflatedFGMRES(PMultigrid(m.elem, m.q_mask, use32=True), tolerance=1e-6, max_iterations=200)
+    solver.setup_preconditioner(K)
+    b = torch.from_numpy(f).to(gpu)
+    solver.A_orthogonalize(K)
+    x = solver.solve(K, b)
+    n0 = solver.iteration_collector.iterations[-1]
+    solver.expand_deflation_basis(x)
+    solver.A_orthogonalize(K)
+    solver.solve(K, b)
+    # the solution is in the A-orthonormal basis: the coarse guess solves the system outright
+    assert solver.iteration_collector.iterations[-1] == 0 < n0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ssr_homo2d_matches_reference_fos(gpu, precision):
+    """2D homogeneous SSR (benchmarks/run_2D_homo_SSR_capture): FoS within 1e-3 of the reference's
+    PETSc run (1.2113326) and lambda history close to its history."""
+    print(f"case: SSR homo2d {precision}", flush=True)
+    h = np.load(GOLD / "ssr_histories.npz")
+    ref = h["homo2d_lambda_hist"]
+    s = cases.make_solver(cases.homo2d(), precision=precision)
+    res = cases.run_ssr(s)
+    print("fos", res.fos, "ref", ref[-1], "steps", len(res.lambda_hist), "newton", res.newton_iterations,
+          "wall", res.wall_time, flush=True)
+    assert abs(res.fos - ref[-1]) < 1e-3
+    assert abs(res.lambda_hist[0] - ref[0]) < 1e-12 and abs(res.lambda_hist[1] - ref[1]) < 1e-12
