@@ -227,8 +227,97 @@ def cpu_baseline(op, E_host, ab, budget_s: float):
     }
 
 
+def run_p4(args):
+    """P4 assembly on the matrix cores: 3D heterogeneous slope (SSR_hetero_ada_L2, 33,935 tets elevated
+    to P4, 24-point rule, four layer materials), return mapping + element matrices (FP64 MFMA) +
+    row gather into the fixed pattern with K_r and FP32 copy."""
+    import torch
+
+    from slope_stability_petsc4py_amd import _lib, cases, workloads
+    from slope_stability_petsc4py_amd import mesh as M
+    from slope_stability_petsc4py_amd.constitutive import ConstitutiveOperator
+    from slope_stability_petsc4py_amd.device import ptr, stream_ptr
+
+    t0 = time.time()
+    d = np.load(ROOT / "tests" / "golden" / "ssr_hetero_L2_tet4.npz", allow_pickle=False)
+    m = M.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
+                               d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,
+                               {"x": [1, 2], "y": [5], "z": [3, 4]}, "P4")
+    op = ConstitutiveOperator.from_mesh(m, workloads.HETERO_LAYERS, Davis_type="B")
+    pat = op.pattern
+    _log(f"P4 mesh+pattern {time.time() - t0:.1f}s: {m.n_elem} tets, {op.n_int} points, {pat.n_rows} dofs, "
+         f"nnz {pat.host.nnz}")
+    E_u, _ = workloads.stratified_strains(min(args.unique, op.n_int), seed=args.seed)
+    reps = -(-op.n_int // E_u.shape[0])
+    op._E.copy_(torch.from_numpy(np.ascontiguousarray(np.tile(E_u, (reps, 1))[: op.n_int])))
+    op.reduction(1.0)
+    K = pat.new_csr(with32=True)
+    st = stream_ptr()
+
+    def k_const():
+        _lib.call("ssx_constitutive_3d", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear), ptr(op.lame),
+                  op.n_int, ptr(op._S), ptr(op._DS), None, None, st)
+
+    def k_ke():
+        pat.element_matrices_p4(op._DS)
+
+    def k_gather():
+        _lib.call("ssx_scatter_element_rows", ptr(pat._ke), pat.nld, ptr(pat.row_slot_ptr), ptr(pat.slot_elem),
+                  ptr(pat.slot_lrow), ptr(pat.slot_pos), ptr(pat.indptr), ptr(pat.row_free), pat.n_rows, 1e-4,
+                  ptr(op.elastic_values), ptr(K.values), ptr(K.values32), pat.max_block_nnz, st)
+
+    step = [("constitutive", k_const), ("ke_mfma", k_ke), ("gather", k_gather)]
+    for _ in range(args.warmup):
+        for _, f in step:
+            f()
+    torch.cuda.synchronize()
+    ev = {n: [] for n, _ in step}
+    a0, b0 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a0.record()
+    for _ in range(args.steps):
+        for name, f in step:
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record()
+            f()
+            b.record()
+            ev[name].append((a, b))
+    b0.record()
+    torch.cuda.synchronize()
+    step_ms = a0.elapsed_time(b0) / args.steps
+    ms = {n: float(np.mean([a.elapsed_time(b) for a, b in lst])) for n, lst in ev.items()}
+    n_e = m.n_elem
+    flops_full = 2.0 * 105 * 105 * 144 * n_e
+    flops_exec = 28 * 36 * (16 * 16 * 4 * 2) * n_e
+    nnz = pat.host.nnz
+    ke_bytes = n_e * 105 * 105 * 8
+    out = {
+        "metric": METRIC + " (P4 MFMA assembly)", "value": round(flops_full / (ms["ke_mfma"] * 1e-3) / 1e12, 2),
+        "unit": "TFLOP/s (effective, full B^T D B)", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(step_ms, 4), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "3D hetero P4 slope (SSR_hetero_ada_L2 elevated to P4)", "n_elem": n_e,
+                   "n_points": int(op.n_int), "n_dof": int(pat.n_rows), "nnz": int(nnz)},
+        "roofline": {"kernel": "ke_mfma", "bound": "mfma", "achieved": round(flops_exec / (ms["ke_mfma"] * 1e-3) / 1e12, 2),
+                     "peak": 78.6, "unit": "TFLOP/s", "frac": round(flops_exec / (ms["ke_mfma"] * 1e-3) / 1e12 / 78.6, 4),
+                     "traffic": None},
+        "kernels": {
+            "constitutive": {"avg_us": round(ms["constitutive"] * 1e3, 1),
+                             "GB_s": round(op.n_int * 416 / (ms["constitutive"] * 1e-3) / 1e9, 1)},
+            "ke_mfma": {"avg_us": round(ms["ke_mfma"] * 1e3, 1), "executed_TFLOP_s":
+                        round(flops_exec / (ms["ke_mfma"] * 1e-3) / 1e12, 2),
+                        "effective_TFLOP_s": round(flops_full / (ms["ke_mfma"] * 1e-3) / 1e12, 2)},
+            "gather": {"avg_us": round(ms["gather"] * 1e3, 1),
+                       "GB_s": round((2 * ke_bytes + nnz * 20) / (ms["gather"] * 1e-3) / 1e9, 1)},
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--config", default="kernel3d", choices=["kernel3d", "p4"],
+                    help="kernel3d: 3D homogeneous P2 slope (default); p4: heterogeneous P4 slope on MFMA")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
@@ -245,6 +334,9 @@ def main():
         print("bench.py: this build measures one GPU only (the boxes here have one GPU); "
               "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
         sys.exit(2)
+    if args.config == "p4":
+        run_p4(args)
+        return
     res = run_gpu(args)
     op = res["op"]
     kern = res["kernels"]
