@@ -13,30 +13,46 @@
 
 namespace {
 
-// Strain-displacement column for local dof (node j, comp d): the nonzero strain rows.
 // 3D strain order [e11 e22 e33 g12 g23 g13]; 2D [e11 e22 g12].
+//
+// Code-generation rule for every kernel in this library: no divergent multi-way branch may choose
+// between addresses (or offsets) that are then dereferenced.  Select VALUES with branch-free
+// arithmetic instead.  hipcc (AMD clang 22, gfx950) has been seen to place the address of one arm
+// of such a branch under the wrong execution mask, leaving lanes of another arm dereferencing an
+// unassigned register (a GPU page fault); the LLVM IR was correct, the machine code was not.
 template <int DIM>
 struct BCol;
 
 template <>
 struct BCol<3> {
     static constexpr int NS = 6;
-    // B^T s for component d of a node with gradient (gx, gy, gz)
-    __device__ static __forceinline__ double bts(int d, double gx, double gy, double gz, const double *s) {
-        if (d == 0) return gx * s[0] + gy * s[3] + gz * s[5];
-        if (d == 1) return gx * s[3] + gy * s[1] + gz * s[4];
-        return gx * s[5] + gy * s[4] + gz * s[2];
-    }
 };
 
 template <>
 struct BCol<2> {
     static constexpr int NS = 3;
-    __device__ static __forceinline__ double bts(int d, double gx, double gy, double, const double *s) {
-        if (d == 0) return gx * s[0] + gy * s[2];
-        return gy * s[1] + gx * s[2];
-    }
 };
+
+// Row ci of B^T for a node with gradient (x, y, z), as coefficients on the strain rows:
+// ci = 0: (x, 0, 0, y, 0, z); ci = 1: (0, y, 0, x, z, 0); ci = 2: (0, 0, z, 0, y, x).
+// Branch-free (0/1 weights, exact), so the same six D entries are read for every ci.
+__device__ __forceinline__ void brow3(int ci, double x, double y, double z, double b[6]) {
+    const double c0 = ci == 0 ? 1.0 : 0.0, c1 = ci == 1 ? 1.0 : 0.0, c2 = ci == 2 ? 1.0 : 0.0;
+    b[0] = c0 * x;
+    b[1] = c1 * y;
+    b[2] = c2 * z;
+    b[3] = c0 * y + c1 * x;
+    b[4] = c1 * z + c2 * y;
+    b[5] = c0 * z + c2 * x;
+}
+
+// 2D: ci = 0: (x, 0, y); ci = 1: (0, y, x).
+__device__ __forceinline__ void brow2(int ci, double x, double y, double b[3]) {
+    const double c0 = ci == 0 ? 1.0 : 0.0, c1 = 1.0 - c0;
+    b[0] = c0 * x;
+    b[1] = c1 * y;
+    b[2] = c0 * y + c1 * x;
+}
 
 // ------------------------------------------------------------------------------------------
 // Strain gather: one lane per quadrature point.
@@ -84,8 +100,9 @@ __global__ void __launch_bounds__(256) k_strain(const double *__restrict__ dphi,
 
 // ------------------------------------------------------------------------------------------
 // Internal force, stage 1: element-local vector f_e[a] = sum_q w_q B_a(q)^T S_q, one lane per
-// (element, local dof).  Stage 2 gathers f_e into rows through the row-slot lists, so every row
-// is written by one lane in a fixed (element) order: deterministic, no atomics.
+// (element, node) producing the node's DIM components from one read of each point's stress.
+// Stage 2 gathers f_e into rows through the row-slot lists, so every row is written by one
+// lane in a fixed (element) order: deterministic, no atomics.
 // ------------------------------------------------------------------------------------------
 template <int DIM, int NP>
 __global__ void __launch_bounds__(256) k_force_elem(const double *__restrict__ dphi, const double *__restrict__ S,
@@ -94,19 +111,32 @@ __global__ void __launch_bounds__(256) k_force_elem(const double *__restrict__ d
     constexpr int NLD = DIM * NP;
     constexpr int NS = BCol<DIM>::NS;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n_elem * NLD) return;
-    const int64_t e = t / NLD;
-    const int a = (int)(t - e * NLD);
-    const int node = a / DIM, comp = a % DIM;
-    double acc = 0.0;
+    if (t >= n_elem * NP) return;
+    const int64_t e = t / NP;
+    const int node = (int)(t - e * NP);
+    double f0 = 0.0, f1 = 0.0, f2 = 0.0;
     for (int q = 0; q < n_q; ++q) {
         const int64_t g = e * n_q + q;
         const double gx = dphi[g * NP + node];
         const double gy = dphi[(n_int + g) * NP + node];
-        const double gz = DIM == 3 ? dphi[(2 * n_int + g) * NP + node] : 0.0;
-        acc += w[g] * BCol<DIM>::bts(comp, gx, gy, gz, S + g * NS);
+        const double wg = w[g];
+        const double *s = S + g * NS;
+        if constexpr (DIM == 3) {
+            const double gz = dphi[(2 * n_int + g) * NP + node];
+            const double s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3], s4 = s[4], s5 = s[5];
+            f0 += wg * (gx * s0 + gy * s3 + gz * s5);
+            f1 += wg * (gx * s3 + gy * s1 + gz * s4);
+            f2 += wg * (gx * s5 + gy * s4 + gz * s2);
+        } else {
+            const double s0 = s[0], s1 = s[1], s2 = s[2];
+            f0 += wg * (gx * s0 + gy * s2);
+            f1 += wg * (gy * s1 + gx * s2);
+        }
     }
-    fe[t] = acc;
+    double *o = fe + e * NLD + DIM * node;
+    o[0] = f0;
+    o[1] = f1;
+    if constexpr (DIM == 3) o[2] = f2;
 }
 
 __global__ void __launch_bounds__(256) k_force_gather(const double *__restrict__ fe, const int32_t *__restrict__ row_slot_ptr,
@@ -172,13 +202,14 @@ __global__ void __launch_bounds__(kRows) k_tangent_rows(
                     double t[NS];
                     if constexpr (DIM == 3) {
                         const double xi = gx[ni], yi = gy[ni], zi = gz[ni];
+                        double b[6];
+                        brow3(ci, xi, yi, zi, b);
 #pragma unroll
                         for (int m = 0; m < 6; ++m) {
                             const double *Dm = D + 6 * m;
-                            double v;
-                            if (ci == 0) v = xi * Dm[0] + yi * Dm[3] + zi * Dm[5];
-                            else if (ci == 1) v = yi * Dm[1] + xi * Dm[3] + zi * Dm[4];
-                            else v = zi * Dm[2] + yi * Dm[4] + xi * Dm[5];
+                            double v = b[0] * Dm[0];
+#pragma unroll
+                            for (int s2 = 1; s2 < 6; ++s2) v += b[s2] * Dm[s2];
                             t[m] = wg * v;
                         }
 #pragma unroll
@@ -190,11 +221,12 @@ __global__ void __launch_bounds__(kRows) k_tangent_rows(
                         }
                     } else {
                         const double xi = gx[ni], yi = gy[ni];
+                        double b[3];
+                        brow2(ci, xi, yi, b);
 #pragma unroll
                         for (int m = 0; m < 3; ++m) {
                             const double *Dm = D + 3 * m;
-                            const double v = ci == 0 ? xi * Dm[0] + yi * Dm[2] : yi * Dm[1] + xi * Dm[2];
-                            t[m] = wg * v;
+                            t[m] = wg * (b[0] * Dm[0] + b[1] * Dm[1] + b[2] * Dm[2]);
                         }
 #pragma unroll
                         for (int j = 0; j < NP; ++j) {
@@ -268,8 +300,8 @@ int launch_strain(const double *dphi, const int32_t *edofs, const double *u, int
 template <int DIM, int NP>
 int launch_force_elem(const double *dphi, const double *S, const double *w, int64_t n_elem, int64_t n_int, int n_q,
                       double *fe, hipStream_t st) {
-    hipLaunchKernelGGL((k_force_elem<DIM, NP>), dim3(ssx::grid_for(n_elem * DIM * NP, 256)), dim3(256), 0, st, dphi,
-                       S, w, n_elem, n_int, n_q, fe);
+    hipLaunchKernelGGL((k_force_elem<DIM, NP>), dim3(ssx::grid_for(n_elem * NP, 256)), dim3(256), 0, st, dphi, S, w,
+                       n_elem, n_int, n_q, fe);
     SSX_CHECK_LAUNCH("ssx_force");
     return SSX_OK;
 }

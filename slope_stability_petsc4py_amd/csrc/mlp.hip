@@ -37,13 +37,19 @@ __global__ void __launch_bounds__(64) k_gemm_bf16(int64_t M, int64_t N, int64_t 
     const int rc = lane & 15, kg = lane >> 4;
     const int64_t ia = i0 + rc, jb = j0 + rc;
     f4 acc = {0.f, 0.f, 0.f, 0.f};
+    // Out-of-range operands: the index is clamped into range (every load is in bounds) and the
+    // value is masked to zero, with no branch around the load.
+    const int64_t iac = ia < M ? ia : M - 1, jbc = jb < N ? jb : N - 1;
+    const short ma = ia < M ? (short)-1 : (short)0, mb = jb < N ? (short)-1 : (short)0;
     for (int64_t k0 = 0; k0 < K; k0 += 16) {
         s4 a, b;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int64_t k = k0 + 4 * kg + t;
-            a[t] = (ia < M && k < K) ? A[ia * a_si + k * a_sk] : (short)0;
-            b[t] = (jb < N && k < K) ? B[k * b_sk + jb * b_sj] : (short)0;
+            const int64_t kc = k < K ? k : K - 1;
+            const short mk = k < K ? (short)-1 : (short)0;
+            a[t] = (short)(A[iac * a_si + kc * a_sk] & ma & mk);
+            b[t] = (short)(B[kc * b_sk + jbc * b_sj] & mb & mk);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }

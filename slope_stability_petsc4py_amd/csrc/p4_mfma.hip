@@ -27,16 +27,26 @@ constexpr int NQ4 = 24;
 constexpr int KDIM = NQ4 * 6;  // 144
 constexpr int TILES = 7;        // ceil(105 / 16)
 
-// Strain row s of B for local dof i = 3*node + d, gradients (gx, gy, gz) of that node.
-__device__ __forceinline__ double brow(int s, int d, double gx, double gy, double gz) {
-    switch (s) {
-        case 0: return d == 0 ? gx : 0.0;
-        case 1: return d == 1 ? gy : 0.0;
-        case 2: return d == 2 ? gz : 0.0;
-        case 3: return d == 0 ? gy : (d == 1 ? gx : 0.0);
-        case 4: return d == 1 ? gz : (d == 2 ? gy : 0.0);
-        default: return d == 0 ? gz : (d == 2 ? gx : 0.0);
-    }
+// Column of B for local dof i = 3*node + d (gradients (gx, gy, gz) of that node), as the six
+// strain-row coefficients, branch-free (0/1 weights): no divergent branch selects an address or
+// an LDS offset anywhere in this file (hipcc for gfx950 has misplaced such selections under the
+// wrong execution mask).
+__device__ __forceinline__ void bcol(int d, double gx, double gy, double gz, double c[6]) {
+    const double c0 = d == 0 ? 1.0 : 0.0, c1 = d == 1 ? 1.0 : 0.0, c2 = d == 2 ? 1.0 : 0.0;
+    c[0] = c0 * gx;
+    c[1] = c1 * gy;
+    c[2] = c2 * gz;
+    c[3] = c0 * gy + c1 * gx;
+    c[4] = c1 * gz + c2 * gy;
+    c[5] = c0 * gz + c2 * gx;
+}
+
+// Entry s of a six-vector, branch-free.
+__device__ __forceinline__ double pick6(int s, const double c[6]) {
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < 6; ++r) v += (s == r ? 1.0 : 0.0) * c[r];
+    return v;
 }
 
 __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dphi, const double *__restrict__ DS,
@@ -66,26 +76,35 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
 #pragma unroll
         for (int tj = 0; tj < TILES; ++tj) acc[tj] = d4{0.0, 0.0, 0.0, 0.0};
         const int ia = 16 * ti + col;
-        const int na = ia / 3, da = ia % 3;
+        // padded rows/columns (>= 105) use a valid node and a zero weight
+        const double va = ia < NLD4 ? 1.0 : 0.0;
+        const int iac = ia < NLD4 ? ia : NLD4 - 1;
+        const int na = iac / 3, da = iac % 3;
         for (int ks = 0; ks < KDIM / 4; ++ks) {
             const int k = 4 * ks + kq;
             const int q = k / 6, s = k % 6;
-            const double a = ia < NLD4 ? brow(s, da, sg[0][q][na], sg[1][q][na], sg[2][q][na]) : 0.0;
+            double ca[6];
+            bcol(da, sg[0][q][na], sg[1][q][na], sg[2][q][na], ca);
+            const double a = va * pick6(s, ca);
             const double *D = sd[q];
+            // row s of wD: D column-major, D[s + 6 t]
+            double drow[6];
+#pragma unroll
+            for (int t = 0; t < 6; ++t) drow[t] = D[s + 6 * t];
 #pragma unroll
             for (int tj = 0; tj < TILES; ++tj) {
                 if (tj < ti) continue;
                 const int jb = 16 * tj + col;
-                double b = 0.0;
-                if (jb < NLD4) {
-                    const int nb = jb / 3, db = jb % 3;
-                    const double gx = sg[0][q][nb], gy = sg[1][q][nb], gz = sg[2][q][nb];
-                    // row s of (wD) B: sum_t wD[s][t] B[t][jb], D column-major D[s + 6 t]
-                    if (db == 0) b = D[s] * gx + D[s + 18] * gy + D[s + 30] * gz;
-                    else if (db == 1) b = D[s + 6] * gy + D[s + 18] * gx + D[s + 24] * gz;
-                    else b = D[s + 12] * gz + D[s + 24] * gy + D[s + 30] * gx;
-                }
-                acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[tj], 0, 0, 0);
+                const double vb = jb < NLD4 ? 1.0 : 0.0;
+                const int jbc = jb < NLD4 ? jb : NLD4 - 1;
+                const int nb = jbc / 3, db = jbc % 3;
+                double cb[6];
+                bcol(db, sg[0][q][nb], sg[1][q][nb], sg[2][q][nb], cb);
+                // row s of (wD) B: sum_t wD[s][t] B[t][jb]
+                double b = drow[0] * cb[0];
+#pragma unroll
+                for (int t = 1; t < 6; ++t) b += drow[t] * cb[t];
+                acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, vb * b, acc[tj], 0, 0, 0);
             }
         }
         // D layout: lane holds rows 4*(lane/16) + r of column lane%16
