@@ -107,14 +107,15 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
                 acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, vb * b, acc[tj], 0, 0, 0);
             }
         }
-        // D layout: lane holds rows 4*(lane/16) + r of column lane%16
+        // accumulator layout of v_mfma_f64_16x16x4f64: register r of lane l holds row
+        // (l / 16) + 4 r of column l % 16
 #pragma unroll
         for (int tj = 0; tj < TILES; ++tj) {
             if (tj < ti) continue;
             const int j = 16 * tj + col;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = 16 * ti + 4 * kq + r;
+                const int i = 16 * ti + kq + 4 * r;
                 if (i < NLD4 && j < NLD4) {
                     out[(int64_t)i * NLD4 + j] = acc[tj][r];
                     if (tj > ti) out[(int64_t)j * NLD4 + i] = acc[tj][r];
