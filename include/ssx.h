@@ -103,9 +103,11 @@ int ssx_axpby_f32(int64_t n, float a, const float *x, float b, float *y, void *s
 int ssx_cast_f64_f32(int64_t n, const double *x, float *y, void *stream);
 int ssx_cast_f32_f64(int64_t n, const float *x, double *y, void *stream);
 
-/* out[j] = V[:, j] . x for j < k (column stride ld): the classical Gram-Schmidt projections of one
- * Arnoldi step (linear/deflated_fgmres.py:261-263, cython/_kernels.pyx:190-192). */
-int ssx_dot_cols(int64_t n, int k, const double *V, int64_t ld, const double *x, double *out, void *stream);
+/* out[j] = V[:, j] . x for j < k (column stride ld, k <= 65535): the classical Gram-Schmidt
+ * projections of one Arnoldi step (linear/deflated_fgmres.py:261-263, cython/_kernels.pyx:190-192).
+ * part: scratch of at least 64 * k doubles (per-chunk partials, summed in a fixed order). */
+int ssx_dot_cols(int64_t n, int k, const double *V, int64_t ld, const double *x, double *part, double *out,
+                 void *stream);
 /* out[0] = x . y, deterministic two-stage reduction; part: >= 1024 doubles of scratch
  * (cython/_kernels.pyx:100-123 `dot`/`norm2`). */
 int ssx_dot(int64_t n, const double *x, const double *y, double *part, double *out, void *stream);

@@ -34,7 +34,7 @@ _SIGS: dict[str, list] = {
     "ssx_axpby_f32": [I64, F32, P, F32, P, P],
     "ssx_cast_f64_f32": [I64, P, P, P],
     "ssx_cast_f32_f64": [I64, P, P, P],
-    "ssx_dot_cols": [I64, I32, P, I64, P, P, P],
+    "ssx_dot_cols": [I64, I32, P, I64, P, P, P, P],
     "ssx_dot": [I64, P, P, P, P, P],
     "ssx_gemv_cols": [I64, I32, P, I64, P, F64, P, P],
     "ssx_cheb_update_f32": [I64, P, P, F32, F32, P, P, P],

@@ -95,18 +95,19 @@ class Solver:
     linear: DeflatedFGMRES
 
 
-def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg") -> Solver:
+def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_refresh: int = 1) -> Solver:
     """precision 'fp64': FP64 preconditioner; 'fp32': FP32 tangent shadow + FP32 preconditioner
-    (the outer FGMRES, residuals and return mapping stay FP64)."""
+    (the outer FGMRES, residuals and return mapping stay FP64).  `coarse_refresh`: the p-multigrid
+    coarse operator is rebuilt every that many preconditioner setups (1 = every Newton iteration)."""
     use32 = precision == "fp32"
     op = ConstitutiveOperator.from_mesh(case.mesh, case.materials, Davis_type=case.davis, fp32_shadow=use32)
     f_field = op.gravity_load(case.mesh)
     f = to_dev(f_field.T.reshape(-1).copy())
     free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
     if pc == "pmg" and case.mesh.elem_type == "P4":
-        prec = PMultigridP4(case.mesh, op, use32=use32)
+        prec = PMultigridP4(case.mesh, op, use32=use32, coarse_refresh=coarse_refresh)
     elif pc == "pmg":
-        prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32)
+        prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32, coarse_refresh=coarse_refresh)
     else:
         prec = ChebyshevJacobi(use32=use32)
     lin = DeflatedFGMRES(prec, tolerance=case.params.linear_tol, max_iterations=case.params.linear_maxit)

@@ -237,10 +237,20 @@ __global__ void __launch_bounds__(kRows) k_tangent_rows(
                     }
                 }
                 const int32_t *pos = slot_pos + (int64_t)s * NLD;
+                int32_t pv[NLD];
 #pragma unroll
-                for (int b = 0; b < NLD; ++b) {
-                    const int32_t p = pos[b];
-                    if (p >= 0) buf[p - v0] += acc[b];
+                for (int b = 0; b < NLD; ++b) pv[b] = pos[b];
+                if constexpr (USE_LDS) {
+                    // constrained columns (p < 0) go to the dummy word buf[L]: no branch per column
+#pragma unroll
+                    for (int b = 0; b < NLD; ++b) {
+                        const int64_t idx = pv[b] >= 0 ? (int64_t)pv[b] - v0 : L;
+                        buf[idx] += acc[b];
+                    }
+                } else {
+#pragma unroll
+                    for (int b = 0; b < NLD; ++b)
+                        if (pv[b] >= 0) buf[pv[b] - v0] += acc[b];
                 }
             }
         }
@@ -262,7 +272,7 @@ int launch_tangent(const double *dphi, const double *DS, const double *w, const 
                    const uint8_t *row_free, int64_t n_rows, int64_t n_int, int n_q, double r, const double *elastic,
                    double *out, float *out32, int64_t max_block_nnz, hipStream_t st) {
     const unsigned grid = ssx::grid_for(n_rows, kRows);
-    const size_t lds_bytes = (size_t)max_block_nnz * sizeof(double);
+    const size_t lds_bytes = ((size_t)max_block_nnz + 1) * sizeof(double);  // + the dummy word
     static int max_lds = -1;
     if (max_lds < 0) {
         int dev = 0, v = 65536;

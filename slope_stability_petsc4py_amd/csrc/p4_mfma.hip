@@ -127,9 +127,12 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
 
 // Row gather of element matrices: one lane per row, slots in element order; LDS row-block tile
 // as in the P2 tangent kernel (USE_LDS) or read-modify-write on the lane's own row in HBM.
+// NLD > 0: compile-time element size, columns handled in register chunks of CH (positions and
+// values loaded as vectors, then branch-free LDS updates with constrained columns sent to the
+// dummy word buf[L]); NLD == 0: any element size.
 constexpr int kRowsKe = 64;
 
-template <bool USE_LDS>
+template <int NLD, bool USE_LDS>
 __global__ void __launch_bounds__(kRowsKe) k_scatter_ke_rows(const double *__restrict__ ke, int nld,
                                                              const int32_t *__restrict__ row_slot_ptr,
                                                              const int32_t *__restrict__ slot_elem,
@@ -152,12 +155,33 @@ __global__ void __launch_bounds__(kRowsKe) k_scatter_ke_rows(const double *__res
         if (!row_free[row]) {
             buf[indptr[row] - v0] = 1.0;
         } else {
-            for (int32_t s = row_slot_ptr[row]; s < row_slot_ptr[row + 1]; ++s) {
-                const double *kr = ke + ((int64_t)slot_elem[s] * nld + slot_lrow[s]) * nld;
-                const int32_t *pos = slot_pos + (int64_t)s * nld;
-                for (int b = 0; b < nld; ++b) {
-                    const int32_t p = pos[b];
-                    if (p >= 0) buf[p - v0] += kr[b];
+            const int32_t s1 = row_slot_ptr[row + 1];
+            for (int32_t s = row_slot_ptr[row]; s < s1; ++s) {
+                if constexpr (NLD > 0 && USE_LDS) {
+                    constexpr int CH = NLD % 15 == 0 ? 15 : NLD;
+                    const double *kr = ke + ((int64_t)slot_elem[s] * NLD + slot_lrow[s]) * NLD;
+                    const int32_t *pos = slot_pos + (int64_t)s * NLD;
+                    for (int b0 = 0; b0 < NLD; b0 += CH) {
+                        int32_t pv[CH];
+                        double kv[CH];
+#pragma unroll
+                        for (int b = 0; b < CH; ++b) {
+                            pv[b] = pos[b0 + b];
+                            kv[b] = kr[b0 + b];
+                        }
+#pragma unroll
+                        for (int b = 0; b < CH; ++b) {
+                            const int64_t idx = pv[b] >= 0 ? (int64_t)pv[b] - v0 : L;
+                            buf[idx] += kv[b];
+                        }
+                    }
+                } else {
+                    const double *kr = ke + ((int64_t)slot_elem[s] * nld + slot_lrow[s]) * nld;
+                    const int32_t *pos = slot_pos + (int64_t)s * nld;
+                    for (int b = 0; b < nld; ++b) {
+                        const int32_t p = pos[b];
+                        if (p >= 0) buf[p - v0] += kr[b];
+                    }
                 }
             }
         }
@@ -171,6 +195,36 @@ __global__ void __launch_bounds__(kRowsKe) k_scatter_ke_rows(const double *__res
         else buf[i] = v;
         if (out32) out32[v0 + i] = (float)v;
     }
+}
+
+template <int NLD>
+int launch_scatter(const double *ke, int nld, const int32_t *row_slot_ptr, const int32_t *slot_elem,
+                   const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr, const uint8_t *row_free,
+                   int64_t n_rows, double r, const double *elastic, double *out, float *out32, int64_t max_block_nnz,
+                   hipStream_t st) {
+    const unsigned grid = ssx::grid_for(n_rows, kRowsKe);
+    const size_t lds_bytes = ((size_t)max_block_nnz + 1) * sizeof(double);  // + the dummy word
+    static int max_lds = -1;
+    if (max_lds < 0) {
+        int dev = 0, v = 65536;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            v = 65536;
+        max_lds = v;
+    }
+    bool lds_ok = max_block_nnz >= 0 && lds_bytes <= (size_t)max_lds;
+    if (lds_ok && lds_bytes > 65536)
+        lds_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scatter_ke_rows<NLD, true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
+    if (lds_ok)
+        hipLaunchKernelGGL((k_scatter_ke_rows<NLD, true>), dim3(grid), dim3(kRowsKe), lds_bytes, st, ke, nld,
+                           row_slot_ptr, slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out,
+                           out32);
+    else
+        hipLaunchKernelGGL((k_scatter_ke_rows<NLD, false>), dim3(grid), dim3(kRowsKe), 0, st, ke, nld, row_slot_ptr,
+                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out, out32);
+    SSX_CHECK_LAUNCH("ssx_scatter_element_rows");
+    return SSX_OK;
 }
 
 }  // namespace
@@ -194,25 +248,15 @@ int ssx_scatter_element_rows(const double *ke, int nld, const int32_t *row_slot_
                              float *out32, int64_t max_block_nnz, void *stream) {
     SSX_REQUIRE(n_rows >= 0 && nld > 0, "ssx_scatter_element_rows: bad sizes");
     if (n_rows == 0) return SSX_OK;
-    const unsigned grid = ssx::grid_for(n_rows, kRowsKe);
-    const size_t lds_bytes = (size_t)max_block_nnz * sizeof(double);
-    int dev = 0, max_lds = 65536;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-        max_lds = 65536;
-    bool lds_ok = max_block_nnz >= 0 && lds_bytes <= (size_t)max_lds;
-    if (lds_ok && lds_bytes > 65536)
-        lds_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scatter_ke_rows<true>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
+    SSX_REQUIRE(ke && row_slot_ptr && indptr && row_free && out, "ssx_scatter_element_rows: null pointer");
     hipStream_t st = ssx::as_stream(stream);
-    if (lds_ok)
-        hipLaunchKernelGGL(k_scatter_ke_rows<true>, dim3(grid), dim3(kRowsKe), lds_bytes, st, ke, nld, row_slot_ptr,
-                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out, out32);
-    else
-        hipLaunchKernelGGL(k_scatter_ke_rows<false>, dim3(grid), dim3(kRowsKe), 0, st, ke, nld, row_slot_ptr,
-                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out, out32);
-    SSX_CHECK_LAUNCH("ssx_scatter_element_rows");
-    return SSX_OK;
+#define CALL(N) launch_scatter<N>(ke, nld, row_slot_ptr, slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, \
+                                  elastic, out, out32, max_block_nnz, st)
+    if (nld == 12) return CALL(12);
+    if (nld == 30) return CALL(30);
+    if (nld == 105) return CALL(105);
+    return CALL(0);
+#undef CALL
 }
 
 }  // extern "C"

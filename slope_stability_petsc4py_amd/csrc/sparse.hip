@@ -65,22 +65,38 @@ __global__ void k_f2d(int64_t n, const float *__restrict__ x, double *__restrict
     if (i < n) y[i] = (double)x[i];
 }
 
-// out[j] = sum_i V[j*ld + i] * x[i] for j < k: one workgroup per column j, deterministic tree
-// reduction (the classical Gram-Schmidt projections of one Arnoldi step in a single launch).
-__global__ void __launch_bounds__(256) k_dot_cols(int64_t n, int k, const double *__restrict__ V, int64_t ld,
-                                                  const double *__restrict__ x, double *__restrict__ out) {
+// out[j] = sum_i V[j*ld + i] * x[i] for j < k (the classical Gram-Schmidt projections of one
+// Arnoldi step): stage 1 is a (chunk, column) grid writing one partial per workgroup, stage 2 sums
+// each column's partials in chunk order -- deterministic, and the grid has k * n_chunk workgroups
+// instead of k.
+constexpr int kDotChunk = 4096;   // vector entries per stage-1 workgroup
+constexpr int kDotMaxChunks = 64;
+
+__global__ void __launch_bounds__(256) k_dot_cols_part(int64_t n, const double *__restrict__ V, int64_t ld,
+                                                       const double *__restrict__ x, int64_t chunk,
+                                                       double *__restrict__ part) {
     __shared__ double red[256];
-    const int j = blockIdx.x;
+    const int c = blockIdx.x, j = blockIdx.y;
     const double *v = V + (int64_t)j * ld;
+    const int64_t i0 = (int64_t)c * chunk;
+    const int64_t i1 = i0 + chunk < n ? i0 + chunk : n;
     double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) acc += v[i] * x[i];
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) acc += v[i] * x[i];
     red[threadIdx.x] = acc;
     __syncthreads();
     for (int s = 128; s > 0; s >>= 1) {
         if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[j] = red[0];
+    if (threadIdx.x == 0) part[(int64_t)j * gridDim.x + c] = red[0];
+}
+
+__global__ void k_dot_cols_sum(int k, int n_chunk, const double *__restrict__ part, double *__restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    double acc = 0.0;
+    for (int c = 0; c < n_chunk; ++c) acc += part[(int64_t)j * n_chunk + c];
+    out[j] = acc;
 }
 
 // Two-stage deterministic dot for long vectors: partials per workgroup, then one final sum.
@@ -192,10 +208,20 @@ int ssx_cast_f32_f64(int64_t n, const float *x, double *y, void *stream) {
     return SSX_OK;
 }
 
-int ssx_dot_cols(int64_t n, int k, const double *V, int64_t ld, const double *x, double *out, void *stream) {
+int ssx_dot_cols(int64_t n, int k, const double *V, int64_t ld, const double *x, double *part, double *out,
+                 void *stream) {
     SSX_REQUIRE(n >= 0 && k >= 0 && ld >= n, "ssx_dot_cols: bad sizes");
+    SSX_REQUIRE(k <= 65535, "ssx_dot_cols: k above 65535 columns");
     if (k == 0) return SSX_OK;
-    hipLaunchKernelGGL(k_dot_cols, dim3(k), dim3(256), 0, ssx::as_stream(stream), n, k, V, ld, x, out);
+    SSX_REQUIRE(V && x && part && out, "ssx_dot_cols: null pointer");
+    int64_t n_chunk = (n + kDotChunk - 1) / kDotChunk;
+    if (n_chunk < 1) n_chunk = 1;
+    if (n_chunk > kDotMaxChunks) n_chunk = kDotMaxChunks;
+    const int64_t chunk = (n + n_chunk - 1) / n_chunk;
+    hipStream_t st = ssx::as_stream(stream);
+    hipLaunchKernelGGL(k_dot_cols_part, dim3((unsigned)n_chunk, (unsigned)k), dim3(256), 0, st, n, V, ld, x,
+                       chunk > 0 ? chunk : 1, part);
+    hipLaunchKernelGGL(k_dot_cols_sum, dim3((unsigned)((k + 63) / 64)), dim3(64), 0, st, k, (int)n_chunk, part, out);
     SSX_CHECK_LAUNCH("ssx_dot_cols");
     return SSX_OK;
 }

@@ -15,9 +15,10 @@ pass (two batched dot launches per step instead of j+1 dependent ones); vectors 
 reference's free-dof system).
 
 Preconditioners
-  * ``PMultigrid``: V-cycle P2 -> P1 (vertex dofs), Chebyshev(Jacobi) smoothing, Galerkin coarse
-    operator P^T A P formed on the device and factorised densely (Cholesky) -- in FP32 (values32)
-    or FP64.  Replaces Hypre BoomerAMG / the manual p-MG shell (linear/solver.py:96-779).
+  * ``PMultigrid``: V-cycle P2 -> P1 (vertex dofs), Chebyshev(Jacobi) smoothing in FP32
+    (values32) or FP64, Galerkin coarse operator P^T A P formed on the device, inverted densely
+    (FP64 Cholesky, explicit inverse; a coarse solve is one matrix-vector product).  Replaces
+    Hypre BoomerAMG / the manual p-MG shell (linear/solver.py:96-779).
   * ``ChebyshevJacobi``: polynomial preconditioner alone.
 """
 
@@ -44,6 +45,13 @@ def _st():
 class _Scratch:
     def __init__(self, dev):
         self.part = torch.empty(1024, dtype=torch.float64, device=dev)
+        self.cols = torch.empty(64 * 160, dtype=torch.float64, device=dev)   # dot_cols partials
+
+    def cols_for(self, k: int) -> torch.Tensor:
+        """Partials scratch of ssx_dot_cols for k columns (64 doubles per column)."""
+        if self.cols.numel() < 64 * k:
+            self.cols = torch.empty(64 * k, dtype=torch.float64, device=self.cols.device)
+        return self.cols
         self.out = torch.empty(128, dtype=torch.float64, device=dev)
 
 
@@ -76,6 +84,7 @@ class IterationCollector:
     solve_time: list = field(default_factory=list)
     preconditioner_time: list = field(default_factory=list)
     orthogonalization_time: list = field(default_factory=list)
+    setup_time: list = field(default_factory=list)
 
     def store(self, it, t_solve, t_pc, t_orth):
         self.iterations.append(int(it))
@@ -94,6 +103,12 @@ class IterationCollector:
 
     def get_total_orthogonalization_time(self):
         return float(sum(self.orthogonalization_time))
+
+    def summary(self) -> dict:
+        return {"solves": len(self.iterations), "iterations": self.get_total_iterations(),
+                "setups": len(self.setup_time), "setup_s": float(sum(self.setup_time)),
+                "solve_s": self.get_total_solve_time(), "preconditioner_s": self.get_total_preconditioner_time(),
+                "orthogonalization_s": self.get_total_orthogonalization_time()}
 
 
 # ---------------------------------------------------------------------------------------------
@@ -121,19 +136,18 @@ class _Level:
         fn = "ssx_inv_diag_f32" if self.use32 else "ssx_inv_diag_f64"
         vals = A.values32 if self.use32 else A.values
         _lib.call(fn, A.n_rows, 0, ptr(A.indptr), ptr(A.indices), ptr(vals), ptr(self.dinv), _st())
-        # power iteration on D^-1 A
+        # power iteration on D^-1 A, norms kept on the device (one host read at the end)
         g = torch.Generator(device="cpu").manual_seed(seed)
         v = torch.rand(self.n, generator=g, dtype=torch.float64).to(self.dinv.device).to(self.dtype)
         v /= torch.linalg.vector_norm(v)
-        lam = 1.0
+        nrm = None
         for _ in range(power_its):
             spmv(A, v, self.r, use32=self.use32)
             self.r.mul_(self.dinv)
-            lam = float(torch.linalg.vector_norm(self.r).item())
-            if lam == 0.0:
-                break
-            v = self.r / lam
-        self.lmax = 1.1 * lam
+            nrm = torch.linalg.vector_norm(self.r)
+            v = self.r / torch.clamp(nrm, min=torch.finfo(self.dtype).tiny)
+        lam = float(nrm.item()) if nrm is not None else 1.0
+        self.lmax = 1.1 * lam if lam > 0.0 else 1.0
 
     def cheb(self, b: torch.Tensor, x: torch.Tensor, degree: int, lo_frac: float, zero_guess: bool):
         """`degree` Chebyshev(Jacobi) steps on A x = b over [lo_frac*lmax, lmax]."""
@@ -310,7 +324,8 @@ class PMultigridP4:
     the dense Galerkin product of the P2 level, factorised by Cholesky (as `PMultigrid`).  Chebyshev
     (Jacobi) smoothing on the P4 and P2 levels, FP32 or FP64."""
 
-    def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1):
+    def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
+                 coarse_refresh: int = 1):
         from .device import DevicePattern
         from .fem import geometry, quadrature_3d
         from .pattern import build_owned_pattern
@@ -334,7 +349,8 @@ class PMultigridP4:
         self.el2 = torch.empty(pat2.nnz, dtype=torch.float64, device=dev)
         self.pat2.tangent(DSel, self.el2)
         self.A2 = self.pat2.new_csr(with32=True)
-        self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac)
+        self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac,
+                                 coarse_refresh=coarse_refresh)
         self.level = None
         self.r = 1e-4
 
@@ -375,19 +391,28 @@ class PMultigridP4:
 
 class PMultigrid:
     """Two-level V-cycle: Chebyshev(Jacobi) pre/post smoothing on the P2 operator, Galerkin coarse
-    operator on the P1 vertex dofs solved by a dense Cholesky factorisation."""
+    operator on the P1 vertex dofs.
+
+    The coarse problem is solved with an explicit inverse of the Galerkin operator (Cholesky in
+    FP64, inverse from it, stored in the level precision), so a coarse solve is one dense
+    matrix-vector product: bandwidth-bound, instead of two latency-bound triangular solves.  The
+    coarse operator is rebuilt on the first setup and then every `coarse_refresh` setups (the fine
+    level always uses the current operator; FGMRES is flexible, so a lagged coarse correction
+    only costs iterations, never correctness)."""
 
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_dtype=torch.float64):
+                 coarse_refresh: int = 1):
         self.use32 = use32
         self.degree = degree
         self.lo_frac = lo_frac
-        self.coarse_dtype = coarse_dtype
+        self.coarse_refresh = max(1, int(coarse_refresh))
+        self._setups = 0
         p_ptr, p_idx, p_w, n_c, c_free = p2_to_p1_prolongation(elem, q_mask)
         self.n_c = n_c
         self.c_free = c_free
         self._p_host = (p_ptr, p_idx, p_w)
         self.level = None
+        self.Ainv = None
         self._dev_ready = False
 
     def _to_device(self, dev, n_f):
@@ -418,20 +443,28 @@ class PMultigrid:
         if self.level is None or self.level.A is not A:
             self.level = _Level(A, self.use32)
         self.level.refresh()
-        # coarse operator (FP64 accumulate), identity on constrained coarse dofs
+        if self.Ainv is None or self._setups % self.coarse_refresh == 0:
+            self._coarse(A, dev)
+        self._setups += 1
+
+    def _coarse(self, A: DeviceCSR, dev) -> None:
+        # coarse operator (FP64 accumulate), identity on constrained coarse dofs; the Cholesky
+        # factorisation reads the lower triangle only
+        self.Ainv = None
         Ac = torch.zeros((self.n_c, self.n_c), dtype=torch.float64, device=dev)
         _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.p_ptr),
                   ptr(self.p_idx), ptr(self.p_w), self.n_c, ptr(Ac), _st())
         Ac[self.cfix, self.cfix] = 1.0
-        Ac = 0.5 * (Ac + Ac.t())
-        L, info = torch.linalg.cholesky_ex(Ac.to(self.coarse_dtype))
+        L, info = torch.linalg.cholesky_ex(Ac)
         if int(info.item()) != 0:
-            # indefinite coarse operator (strong softening): fall back to a shifted factorisation
+            # indefinite coarse operator (strong softening): shifted factorisation
             shift = 1e-8 * float(Ac.diagonal().abs().max().item())
-            L = torch.linalg.cholesky(Ac.to(self.coarse_dtype) + shift * torch.eye(self.n_c, device=dev,
-                                                                                   dtype=self.coarse_dtype))
-        self.Lc = L
+            Ac.diagonal().add_(shift)
+            L = torch.linalg.cholesky(Ac)
         del Ac
+        Ainv = torch.cholesky_inverse(L)
+        del L
+        self.Ainv = Ainv.to(torch.float32) if self.use32 else Ainv
 
     def cycle(self, b: torch.Tensor, x: torch.Tensor) -> None:
         """One V-cycle on A x = b in the level precision (x overwritten)."""
@@ -441,8 +474,7 @@ class PMultigrid:
         axpby(1.0, b, 0.0, res)
         spmv(Lv.A, x, res, alpha=-1.0, beta=1.0, use32=self.use32)          # res = b - A x
         spmv(self.PT, res, self.rc, use32=self.use32)                          # restrict
-        ec = torch.cholesky_solve(self.rc.to(self.coarse_dtype)[:, None], self.Lc)[:, 0]
-        self.xc.copy_(ec.to(self.xc.dtype))
+        torch.mv(self.Ainv, self.rc, out=self.xc)                              # coarse solve
         spmv(self.P, self.xc, x, alpha=1.0, beta=1.0, use32=self.use32)       # prolong + correct
         Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=False)          # post-smooth
 
@@ -537,7 +569,10 @@ class DeflatedFGMRES:
         if self._scr is None:
             self._scr = _Scratch(A.indptr.device)
         self.pc.setup(A)
+        if A.values.is_cuda:
+            torch.cuda.synchronize()
         self._pc_setup_time = perf_counter() - t0
+        self.iteration_collector.setup_time.append(self._pc_setup_time)
 
     def A_orthogonalize(self, A: DeviceCSR):
         """A-orthonormalise the basis newest-first, dropping columns with |v^T A v| <= eps
@@ -560,7 +595,7 @@ class DeflatedFGMRES:
             v = B[idx].clone()
             spmv(A, v, av)
             if cnt:
-                _lib.call("ssx_dot_cols", n, cnt, ptr(Wbuf), n, ptr(av), ptr(coeff), _st())
+                _lib.call("ssx_dot_cols", n, cnt, ptr(Wbuf), n, ptr(av), ptr(scr.cols_for(cnt)), ptr(coeff), _st())
                 c = coeff[:cnt].cpu().numpy() * np.asarray(signs)
                 coeff[:cnt].copy_(torch.from_numpy(c))
                 _lib.call("ssx_gemv_cols", n, cnt, ptr(Wbuf), n, ptr(coeff), -1.0, ptr(v), _st())
@@ -596,7 +631,7 @@ class DeflatedFGMRES:
         x = torch.zeros(n, dtype=torch.float64, device=dev)
         if k:
             h = torch.empty(k, dtype=torch.float64, device=dev)
-            _lib.call("ssx_dot_cols", n, k, ptr(W), n, ptr(b), ptr(h), _st())
+            _lib.call("ssx_dot_cols", n, k, ptr(W), n, ptr(b), ptr(scr.cols_for(k)), ptr(h), _st())
             _lib.call("ssx_gemv_cols", n, k, ptr(W), n, ptr(h), 1.0, ptr(x), _st())
         r = b.clone()
         spmv(A, x, r, alpha=-1.0, beta=1.0)
@@ -607,6 +642,7 @@ class DeflatedFGMRES:
             self.iteration_collector.store(0, perf_counter() - t0, 0.0, 0.0)
             self.last_residual_history = np.array(hist)
             return x
+        scr.cols_for(max(m + 1, k))
         V = torch.empty((m + 1, n), dtype=torch.float64, device=dev)
         Z = torch.empty((m, n), dtype=torch.float64, device=dev)
         H = np.zeros((m + 1, m))
@@ -624,14 +660,14 @@ class DeflatedFGMRES:
             t_pc += perf_counter() - tp
             to = perf_counter()
             if k:
-                _lib.call("ssx_dot_cols", n, k, ptr(AW), n, ptr(Z[j]), ptr(cbuf), _st())
+                _lib.call("ssx_dot_cols", n, k, ptr(AW), n, ptr(Z[j]), ptr(scr.cols), ptr(cbuf), _st())
                 _lib.call("ssx_gemv_cols", n, k, ptr(W), n, ptr(cbuf), -1.0, ptr(Z[j]), _st())
             u = V[j + 1]
             spmv(A, Z[j], u)
-            _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(hbuf), _st())
+            _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf), _st())
             _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf), -1.0, ptr(u), _st())
             for _ in range(self.reorth_passes):
-                _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(hbuf2), _st())
+                _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf2), _st())
                 _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf2), -1.0, ptr(u), _st())
                 hbuf[: j + 1] += hbuf2[: j + 1]
             hn = np.sqrt(max(dot(u, u, scr), 0.0))

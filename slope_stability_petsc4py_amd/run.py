@@ -26,6 +26,8 @@ def main(argv=None):
     ap.add_argument("--h", type=float, default=None)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--step-max", type=int, default=None)
+    ap.add_argument("--coarse-refresh", type=int, default=1,
+                    help="rebuild the multigrid coarse operator every N preconditioner setups")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
@@ -37,7 +39,7 @@ def main(argv=None):
         case = cases.slope3d(h=a.h or 2.0, nz=a.nz or 10, hetero=a.case.endswith("hetero"))
     if a.step_max:
         case.params.step_max = a.step_max
-    s = cases.make_solver(case, precision=a.precision, pc=a.pc)
+    s = cases.make_solver(case, precision=a.precision, pc=a.pc, coarse_refresh=a.coarse_refresh)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
 
@@ -54,6 +56,7 @@ def main(argv=None):
                "wall_time_s": res.wall_time, "setup_s": t_setup, "newton_iterations": res.newton_iterations,
                "linear_iterations": res.linear_iterations, "attempts": res.attempts,
                "n_dof": int(s.op.pattern.n_rows), "n_elem": int(case.mesh.n_elem),
+               "coarse_refresh": a.coarse_refresh, "linear_solver": s.linear.iteration_collector.summary(),
                "timings": s.op.get_total_time()}
     else:
         res = cases.run_ll(s, progress=prog)

@@ -210,8 +210,16 @@ def test_vector_kernels(gpu):
     V = torch.from_numpy(rng.normal(size=(k, n))).to(dev)
     x = torch.from_numpy(rng.normal(size=n)).to(dev)
     out = torch.empty(k, dtype=torch.float64, device=dev)
-    _lib.call("ssx_dot_cols", n, k, V.data_ptr(), n, x.data_ptr(), out.data_ptr(), torch.cuda.current_stream().cuda_stream)
+    cols = torch.empty(64 * k, dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream().cuda_stream
+    _lib.call("ssx_dot_cols", n, k, V.data_ptr(), n, x.data_ptr(), cols.data_ptr(), out.data_ptr(), st)
     np.testing.assert_allclose(out.cpu().numpy(), V.cpu().numpy() @ x.cpu().numpy(), rtol=1e-12)
+    # deterministic: same bits on a second call; small n (one chunk) and one column
+    out2 = torch.empty_like(out)
+    _lib.call("ssx_dot_cols", n, k, V.data_ptr(), n, x.data_ptr(), cols.data_ptr(), out2.data_ptr(), st)
+    assert torch.equal(out, out2)
+    _lib.call("ssx_dot_cols", 5, 1, V.data_ptr(), n, x.data_ptr(), cols.data_ptr(), out2.data_ptr(), st)
+    np.testing.assert_allclose(out2[0].item(), float(V[0, :5].cpu().numpy() @ x[:5].cpu().numpy()), rtol=1e-14)
     part = torch.empty(1024, dtype=torch.float64, device=dev)
     d = torch.empty(1, dtype=torch.float64, device=dev)
     _lib.call("ssx_dot", n, x.data_ptr(), x.data_ptr(), part.data_ptr(), d.data_ptr(), torch.cuda.current_stream().cuda_stream)
