@@ -46,13 +46,13 @@ class _Scratch:
     def __init__(self, dev):
         self.part = torch.empty(1024, dtype=torch.float64, device=dev)
         self.cols = torch.empty(64 * 160, dtype=torch.float64, device=dev)   # dot_cols partials
+        self.out = torch.empty(128, dtype=torch.float64, device=dev)
 
     def cols_for(self, k: int) -> torch.Tensor:
         """Partials scratch of ssx_dot_cols for k columns (64 doubles per column)."""
         if self.cols.numel() < 64 * k:
             self.cols = torch.empty(64 * k, dtype=torch.float64, device=self.cols.device)
         return self.cols
-        self.out = torch.empty(128, dtype=torch.float64, device=dev)
 
 
 def dot(x: torch.Tensor, y: torch.Tensor, scr: _Scratch) -> float:
