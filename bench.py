@@ -58,7 +58,7 @@ def build_workload(h: float, nz: int, n_unique: int, seed: int):
     return m, op, E, br
 
 
-def algorithmic_bytes(op) -> dict:
+def algorithmic_bytes(op, fold: bool = True) -> dict:
     """Per-launch algorithmic HBM bytes of each kernel of the step."""
     n = op.n_int
     pat = op.pattern
@@ -69,8 +69,9 @@ def algorithmic_bytes(op) -> dict:
         "constitutive": n * (48 + 32 + 48 + 288),
         # S 48 + dphi 3*n_p*8 + w 8 per point in; fe out; then fe in + F out
         "force": n * (48 + 24 * n_p + 8) + pat.n_elem * pat.nld * 8 * 2 + pat.n_rows * 8,
-        # DS 288 + dphi 3*n_p*8 + w 8 per point in; K_el nnz*8 in; K_r nnz*8 + FP32 nnz*4 out
-        "tangent": n * (288 + 24 * n_p + 8) + nnz * (8 + 8 + 4),
+        # DS 288 + dphi 3*n_p*8 + w 8 (+ shear, lame 16 when K_r is folded into the point tangents,
+        # else the assembled K_el nnz*8) in; K_r nnz*8 + FP32 copy nnz*4 out
+        "tangent": n * (288 + 24 * n_p + 8) + nnz * (8 + 4) + (n * 16 if fold else nnz * 8),
         # dphi + u gather (3*n_p*8 per element, counted once) + E out
         "strain": n * (24 * n_p + 48) + pat.n_elem * pat.nld * 8,
         # CSR FP64: values 8 + index 4 per nnz, x 8 (once) + y 8 + indptr 4 per row
@@ -101,8 +102,11 @@ def run_gpu(args):
     def k_force():
         pat.force(S, fe, F)
 
+    mat = (op.shear, op.lame) if (pat.ns is not None and args.tangent == "nodes") else None
+    el = None if mat is not None else op.elastic_values
+
     def k_tangent():
-        pat.tangent(DS, K.values, K.values32, r=r, elastic=op.elastic_values)
+        pat.tangent(DS, K.values, K.values32, r=r, elastic=el, mat=mat)
 
     u = torch.from_numpy(np.random.default_rng(1).normal(0, 1e-3, pat.n_rows)).to(dev)
     E_tmp = torch.empty_like(op._E)
@@ -150,7 +154,7 @@ def run_gpu(args):
             b.record()
             ev[name].append((a, b))
     torch.cuda.synchronize()
-    ab = algorithmic_bytes(op)
+    ab = algorithmic_bytes(op, fold=mat is not None)
     kern = {}
     for name, lst in ev.items():
         ms = float(np.mean([a.elapsed_time(b) for a, b in lst]))
@@ -327,8 +331,9 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--tangent", default="rows", choices=["rows", "elem"],
-                    help="P2 tangent path: fused row kernel or element blocks + row gather")
+    ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows", "elem"],
+                    help="P2 tangent path: element matrices + node gather (default), fused row kernel, or "
+                         "element blocks + row gather")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the boxes here have one GPU); "

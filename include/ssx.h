@@ -129,9 +129,31 @@ int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 
 /* P4 tetrahedra (35 nodes, 24 points): element tangent matrices ke (n_elem x 105 x 105, row-major,
  * row = local dof 3*node+comp) = sum_q B^T (w D) B on the FP64 matrix cores (v_mfma_f64_16x16x4).
+ * With shear/lame (per point, both or neither) D is the regularised (1-r) DS + r D_el, D_el the
+ * elastic tangent (lame vol vol^T + 2 G ident), which folds K_r = (1-r) K + r K_el into the element
+ * matrices (constitutive/problem.py:1595-1644).
  * Replaces the element stage of assemble_tangent_values_3d_p2_c (cython/assemble_tangent_values_3d.c:45-94). */
-int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double *ke,
-                            void *stream);
+int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double r,
+                            const double *shear, const double *lame, double *ke, void *stream);
+
+/* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p, row-major), one
+ * wavefront per element; optional K_r fold exactly as ssx_element_matrices_p4.  Replaces the
+ * element stage of cython/assemble_tangent_values_3d.c:13-109. */
+int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
+                            double r, const double *shear, const double *lame, double *ke, void *stream);
+
+/* Node gather of element matrices into the fixed CSR pattern: one wavefront per mesh node sums the
+ * node's dim rows over its elements (node slots ns_ptr/ns_elem/ns_lnode, element order) at the
+ * int16 column offsets ns_off (n_slots x nld, relative to the row start, -1 = constrained column),
+ * sets constrained rows to their unit diagonal, optionally blends (1-r) K + r elastic, and writes
+ * FP64 values and an optional FP32 copy.  max_len = longest free row (< 32768).  dim 3, nld 12/30/105.
+ * Together with ssx_element_matrices_3d / _p4 replaces assemble_tangent_values_3d_rows_c
+ * (cython/assemble_tangent_values_3d.c:111-214) and the constrained-diagonal step of
+ * assemble_owned_tangent_values (fem/distributed_tangent.py:1559-1612). */
+int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_ptr, const int32_t *ns_elem,
+                         const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
+                         const uint8_t *row_free, int64_t n_nodes, int max_len, double r, const double *elastic,
+                         double *out, float *out32, void *stream);
 
 /* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p) by 3x3 node
  * blocks on the vector ALUs (one lane per node pair i <= j, mirrored).  Same reference stage as

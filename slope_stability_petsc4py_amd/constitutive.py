@@ -305,7 +305,11 @@ class ConstitutiveOperator:
         K = self._owned_regularized_mat
         self.last_regularization = float(r)
         t0 = perf_counter()
-        self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), elastic=self.elastic_values)
+        if self.pattern.ns is not None:
+            # K_r folded into the point tangents: (1-r) DS + r D_el (same operator, no K_el read)
+            self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), mat=(self.shear, self.lame))
+        else:
+            self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), elastic=self.elastic_values)
         self.time_build_tangent_local.append(perf_counter() - t0)
         return K
 

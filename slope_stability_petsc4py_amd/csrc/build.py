@@ -26,6 +26,7 @@ SOURCES = [
     "p4_mfma.hip",
     "mlp.hip",
     "elem_blocks.hip",
+    "node_assembly.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -1,0 +1,293 @@
+// Two-phase tangent assembly for tetrahedra on gfx950:
+//
+//   phase 1 (k_ke_3d)        one wavefront per element computes the element matrix
+//                            K_e = sum_q w_q B_q^T D_q B_q (dim*n_p square, row-major) once;
+//   phase 2 (k_gather_nodes) one wavefront per mesh node gathers the node's dim rows from the
+//                            element matrices of the elements around it, in element order, into an
+//                            LDS copy of those rows, then writes them to the fixed CSR pattern with
+//                            one coalesced pass (FP64 values and the FP32 copy).
+//
+// The regularised operator K_r = (1-r) K_t + r K_el is linear in D, so phase 1 can fold it into
+// the point tangent: D_r = (1-r) DS + r D_el with D_el = lame * vol vol^T + 2 G * ident (the elastic
+// tangent of the return mapping).  That replaces the read of the assembled elastic values in the
+// blend; the blend form (phase 2 reading K_el) is kept for element matrices made elsewhere.
+//
+// Free rows of one node share their column list (the pattern rule: sorted unique free columns of
+// every element touching the node), so the CSR positions of an element's columns are stored once
+// per (node, element) pair as int16 offsets from the row start: n_p*dim*2 bytes instead of
+// dim*(n_p*dim*4) bytes of per-row positions.  Constrained rows hold only their diagonal (= 1).
+//
+// Summation order of a CSR entry: element matrices are summed over the node's elements in element
+// order, each element matrix being the sum over its points -- the order of the reference's `rows`
+// kernel (cython/assemble_tangent_values_3d.c:111-214).  Every LDS update of one element completes
+// (barrier) before the next element's updates start, so results are bitwise reproducible.
+//
+// Reference behaviour: cython/assemble_tangent_values_3d.c:13-109 (element matrices), :111-214
+// (rows); fem/distributed_tangent.py:1559-1612 (constrained diagonal = 1); constitutive/problem.py:
+// 1595-1644 (K_r on the same pattern).
+#include "ssx_common.h"
+
+namespace {
+
+// Row a = 3*node + c of B^T for a node with gradient (x, y, z), as coefficients on the strain
+// rows [e11 e22 e33 g12 g23 g13]; branch-free (0/1 weights): the same code for every c.
+__device__ __forceinline__ void brow(int c, double x, double y, double z, double b[6]) {
+    const double c0 = c == 0 ? 1.0 : 0.0, c1 = c == 1 ? 1.0 : 0.0, c2 = c == 2 ? 1.0 : 0.0;
+    b[0] = c0 * x;
+    b[1] = c1 * y;
+    b[2] = c2 * z;
+    b[3] = c0 * y + c1 * x;
+    b[4] = c1 * z + c2 * y;
+    b[5] = c0 * z + c2 * x;
+}
+
+// ------------------------------------------------------------------------------------------
+// Phase 1: element matrices of P1/P2 tetrahedra.  Lane (h, a): a = lane % 32 is the element row
+// (a < 3*NP), h = lane / 32 selects the column half (nodes NH*h .. NH*h + NH - 1).  Per point the
+// lane forms t = w (D_r b_a) (six values) and adds t . B for its 3*NH columns.  The element's point
+// data (D, w, material) is the same for every lane of the wavefront.
+// ------------------------------------------------------------------------------------------
+template <int NP>
+__global__ void __launch_bounds__(64) k_ke_3d(const double *__restrict__ dphi, const double *__restrict__ DS,
+                                              const double *__restrict__ w, int64_t n_elem, int n_q, double r,
+                                              const double *__restrict__ shear, const double *__restrict__ lame,
+                                              double *__restrict__ ke) {
+    constexpr int NLD = 3 * NP;
+    constexpr int NH = NP / 2;
+    static_assert(NP % 2 == 0 && NLD <= 32, "two column halves of an element with at most 32 rows");
+    const int64_t e = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int h = lane >> 5;
+    const int a_raw = lane & 31;
+    const bool valid = a_raw < NLD;
+    const int a = valid ? a_raw : NLD - 1;  // clamped row: in-bounds reads, store masked
+    const int ni = a / 3, ci = a - 3 * ni;
+    const int64_t n_int = n_elem * n_q;
+    const bool fold = shear != nullptr;
+    const double om = fold ? 1.0 - r : 1.0;
+    const double rr = fold ? r : 0.0;
+    double acc[3 * NH];
+#pragma unroll
+    for (int k = 0; k < 3 * NH; ++k) acc[k] = 0.0;
+    for (int q = 0; q < n_q; ++q) {
+        const int64_t g = e * n_q + q;
+        const double *gx = dphi + g * NP;
+        const double *gy = dphi + (n_int + g) * NP;
+        const double *gz = dphi + (2 * n_int + g) * NP;
+        const double *D = DS + g * 36;
+        const double wg = w[g];
+        double b[6];
+        brow(ci, gx[ni], gy[ni], gz[ni], b);
+        // D b (D column-major: D[s + 6 m]); D_el b = lame tr(b) vol + 2 G ident b
+        double t[6];
+#pragma unroll
+        for (int m = 0; m < 6; ++m) {
+            double v = b[0] * D[6 * m];
+#pragma unroll
+            for (int s = 1; s < 6; ++s) v += b[s] * D[s + 6 * m];
+            t[m] = v;
+        }
+        if (fold) {
+            const double G = shear[g], L = lame[g];
+            const double tr = L * (b[0] + b[1] + b[2]);
+#pragma unroll
+            for (int m = 0; m < 6; ++m) {
+                const double del = (m < 3 ? tr + 2.0 * G * b[m] : G * b[m]);
+                t[m] = om * t[m] + rr * del;
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < 6; ++m) t[m] *= wg;
+#pragma unroll
+        for (int k = 0; k < NH; ++k) {
+            const int j = NH * h + k;
+            const double xj = gx[j], yj = gy[j], zj = gz[j];
+            acc[3 * k + 0] += t[0] * xj + t[3] * yj + t[5] * zj;
+            acc[3 * k + 1] += t[3] * xj + t[1] * yj + t[4] * zj;
+            acc[3 * k + 2] += t[5] * xj + t[4] * yj + t[2] * zj;
+        }
+    }
+    if (valid) {
+        double *o = ke + ((int64_t)e * NLD + a) * NLD + 3 * NH * h;
+#pragma unroll
+        for (int k = 0; k < 3 * NH; ++k) o[k] = acc[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Phase 2: node gather.  One wavefront per node; LDS holds the node's DIM rows (free rows share
+// one length L) plus a dummy word that absorbs the updates of constrained columns and rows.  Each
+// element contributes its DIM contiguous rows DIM*li .. DIM*li + DIM-1 (DIM*NLD values, read
+// coalesced); lane p handles value p (and p + 64, ...), i.e. row c = p / NLD, column b = p % NLD.
+// Within one element the targets are distinct (distinct columns); elements are applied one after
+// the other with a barrier in between, and the next element's values are loaded before the
+// barrier (software pipelining).
+// ------------------------------------------------------------------------------------------
+template <int DIM, int NLD>
+__global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ ke, const int32_t *__restrict__ ns_ptr,
+                                                     const int32_t *__restrict__ ns_elem,
+                                                     const uint8_t *__restrict__ ns_lnode,
+                                                     const int16_t *__restrict__ ns_off,
+                                                     const int32_t *__restrict__ indptr,
+                                                     const uint8_t *__restrict__ row_free, double r,
+                                                     const double *__restrict__ elastic, double *__restrict__ out,
+                                                     float *__restrict__ out32) {
+    constexpr int NV = DIM * NLD;
+    constexpr int NPASS = (NV + 63) / 64;
+    extern __shared__ double acc[];
+    const int64_t node = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t r0 = DIM * node;
+    int fr[DIM];
+    int64_t st[DIM];
+    int L = 0;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        fr[c] = row_free[r0 + c];
+        st[c] = indptr[r0 + c];
+        const int len = indptr[r0 + c + 1] - indptr[r0 + c];
+        L = fr[c] ? len : L;
+    }
+    const int dummy = DIM * L;
+    for (int i = lane; i <= dummy; i += 64) acc[i] = 0.0;
+
+    // per-pass constant part of the LDS target: c*L for a free row, the dummy word otherwise
+    int rowbase[NPASS], col[NPASS];
+    bool live[NPASS];
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+        const int p = lane + 64 * k;
+        live[k] = p < NV;
+        const int pc = live[k] ? p : 0;
+        const int c = pc / NLD;
+        col[k] = pc - c * NLD;
+        int frc = 0;
+#pragma unroll
+        for (int cc = 0; cc < DIM; ++cc) frc += (c == cc) ? fr[cc] : 0;
+        rowbase[k] = (live[k] && frc) ? c * L : -1;
+    }
+
+    const int32_t s0 = ns_ptr[node], s1 = ns_ptr[node + 1];
+    double v[NPASS];
+    int o[NPASS];
+    auto load = [&](int32_t s, double *vv, int *oo) {
+        const int64_t e = ns_elem[s];
+        const int li = ns_lnode[s];
+        const double *kb = ke + (e * NLD + (int64_t)DIM * li) * NLD;
+        const int16_t *off = ns_off + (int64_t)s * NLD;
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k) {
+            const int p = live[k] ? lane + 64 * k : 0;
+            vv[k] = kb[p];
+            oo[k] = off[col[k]];
+        }
+    };
+    if (s0 < s1) load(s0, v, o);
+    __syncthreads();
+    for (int32_t s = s0; s < s1; ++s) {
+        double vn[NPASS];
+        int on[NPASS];
+        load(s + 1 < s1 ? s + 1 : s, vn, on);
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k) {
+            const int idx = (rowbase[k] >= 0 && o[k] >= 0) ? rowbase[k] + o[k] : dummy;
+            acc[idx] += v[k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NPASS; ++k) {
+            v[k] = vn[k];
+            o[k] = on[k];
+        }
+    }
+    const double om = 1.0 - r;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        if (fr[c]) {
+            for (int i = lane; i < L; i += 64) {
+                double val = acc[c * L + i];
+                if (elastic) val = om * val + r * elastic[st[c] + i];
+                out[st[c] + i] = val;
+                if (out32) out32[st[c] + i] = (float)val;
+            }
+        } else if (lane == 0) {
+            out[st[c]] = 1.0;  // constrained row: its diagonal only
+            if (out32) out32[st[c]] = 1.0f;
+        }
+    }
+}
+
+template <int DIM, int NLD>
+int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_elem, const uint8_t *ns_lnode,
+                  const int16_t *ns_off, const int32_t *indptr, const uint8_t *row_free, int64_t n_nodes,
+                  int max_len, double r, const double *elastic, double *out, float *out32, hipStream_t st) {
+    const size_t lds = ((size_t)DIM * max_len + 1) * sizeof(double);
+    if (lds > 65536) {
+        static int max_lds = -1;
+        if (max_lds < 0) {
+            int dev = 0, v = 65536;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+                v = 65536;
+            max_lds = v;
+        }
+        SSX_REQUIRE(lds <= (size_t)max_lds, "ssx_gather_node_rows: rows of %d entries exceed the LDS", max_len);
+        if (hipFuncSetAttribute(reinterpret_cast<const void *>(&k_gather_nodes<DIM, NLD>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess) {
+            ssx::set_error("ssx_gather_node_rows: cannot raise the LDS limit to %zu bytes", lds);
+            return SSX_ERUNTIME;
+        }
+    }
+    hipLaunchKernelGGL((k_gather_nodes<DIM, NLD>), dim3((unsigned)n_nodes), dim3(64), lds, st, ke, ns_ptr, ns_elem,
+                       ns_lnode, ns_off, indptr, row_free, r, elastic, out, out32);
+    SSX_CHECK_LAUNCH("ssx_gather_node_rows");
+    return SSX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
+                            double r, const double *shear, const double *lame, double *ke, void *stream) {
+    SSX_REQUIRE(n_elem >= 0 && n_q > 0, "ssx_element_matrices_3d: bad sizes");
+    if (n_elem == 0) return SSX_OK;
+    SSX_REQUIRE(dphi && DS && w && ke, "ssx_element_matrices_3d: null pointer");
+    SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_matrices_3d: shear and lame go together");
+    SSX_REQUIRE(n_elem <= 0x7fffffff, "ssx_element_matrices_3d: too many elements for one launch");
+    hipStream_t st = ssx::as_stream(stream);
+    if (n_p == 10)
+        hipLaunchKernelGGL((k_ke_3d<10>), dim3((unsigned)n_elem), dim3(64), 0, st, dphi, DS, w, n_elem, n_q, r, shear,
+                           lame, ke);
+    else if (n_p == 4)
+        hipLaunchKernelGGL((k_ke_3d<4>), dim3((unsigned)n_elem), dim3(64), 0, st, dphi, DS, w, n_elem, n_q, r, shear,
+                           lame, ke);
+    else {
+        ssx::set_error("ssx_element_matrices_3d: n_p must be 4 or 10");
+        return SSX_EINVAL;
+    }
+    SSX_CHECK_LAUNCH("ssx_element_matrices_3d");
+    return SSX_OK;
+}
+
+int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_ptr, const int32_t *ns_elem,
+                         const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
+                         const uint8_t *row_free, int64_t n_nodes, int max_len, double r, const double *elastic,
+                         double *out, float *out32, void *stream) {
+    SSX_REQUIRE(n_nodes >= 0 && max_len >= 0 && max_len < 32768, "ssx_gather_node_rows: bad sizes");
+    if (n_nodes == 0) return SSX_OK;
+    SSX_REQUIRE(ke && ns_ptr && ns_elem && ns_lnode && ns_off && indptr && row_free && out,
+                "ssx_gather_node_rows: null pointer");
+    SSX_REQUIRE(n_nodes <= 0x7fffffff, "ssx_gather_node_rows: too many nodes for one launch");
+    hipStream_t st = ssx::as_stream(stream);
+#define CALL(D, N) \
+    launch_gather<D, N>(ke, ns_ptr, ns_elem, ns_lnode, ns_off, indptr, row_free, n_nodes, max_len, r, elastic, out, out32, st)
+    if (dim == 3 && nld == 12) return CALL(3, 12);
+    if (dim == 3 && nld == 30) return CALL(3, 30);
+    if (dim == 3 && nld == 105) return CALL(3, 105);
+#undef CALL
+    ssx::set_error("ssx_gather_node_rows: unsupported element (dim=%d, nld=%d)", dim, nld);
+    return SSX_EINVAL;
+}
+
+}  // extern "C"

@@ -51,7 +51,8 @@ __device__ __forceinline__ double pick6(int s, const double c[6]) {
 
 __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dphi, const double *__restrict__ DS,
                                                    const double *__restrict__ w, int64_t n_elem, int64_t n_int,
-                                                   double *__restrict__ ke) {
+                                                   double r, const double *__restrict__ shear,
+                                                   const double *__restrict__ lame, double *__restrict__ ke) {
     __shared__ double sg[3][NQ4][NP4];  // gradients of the element's basis functions
     __shared__ double sd[NQ4][36];      // w_q * DS_q (column-major 6x6)
     const int64_t e = blockIdx.x;
@@ -62,9 +63,20 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
         const int c = t / (NQ4 * NP4), rem = t % (NQ4 * NP4);
         sg[c][rem / NP4][rem % NP4] = dphi[((int64_t)c * n_int + g0) * NP4 + rem];
     }
+    // w_q * D_q, or w_q * ((1 - r) DS_q + r D_el) with the elastic tangent
+    // D_el = lame * vol vol^T + 2 G * ident (ident = diag(1, 1, 1, 1/2, 1/2, 1/2)) when the
+    // regularisation is folded in (shear != nullptr)
+    const bool fold = shear != nullptr;
     for (int t = lane; t < NQ4 * 36; t += 64) {
-        const int q = t / 36;
-        sd[q][t % 36] = w[g0 + q] * DS[(g0 + q) * 36 + (t % 36)];
+        const int q = t / 36, k = t % 36;
+        const int i = k % 6, j = k / 6;
+        double d = DS[(g0 + q) * 36 + k];
+        if (fold) {
+            const double G = shear[g0 + q], L = lame[g0 + q];
+            const double del = (i < 3 && j < 3 ? L : 0.0) + (i == j ? (i < 3 ? 2.0 * G : G) : 0.0);
+            d = (1.0 - r) * d + r * del;
+        }
+        sd[q][k] = w[g0 + q] * d;
     }
     __syncthreads();
 
@@ -231,13 +243,14 @@ int launch_scatter(const double *ke, int nld, const int32_t *row_slot_ptr, const
 
 extern "C" {
 
-int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double *ke,
-                            void *stream) {
+int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double r,
+                            const double *shear, const double *lame, double *ke, void *stream) {
     SSX_REQUIRE(n_elem >= 0, "ssx_element_matrices_p4: n_elem < 0");
     if (n_elem == 0) return SSX_OK;
     SSX_REQUIRE(dphi && DS && w && ke, "ssx_element_matrices_p4: null pointer");
+    SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_matrices_p4: shear and lame go together");
     hipLaunchKernelGGL(k_ke_p4_mfma, dim3((unsigned)n_elem), dim3(64), 0, ssx::as_stream(stream), dphi, DS, w, n_elem,
-                       n_elem * NQ4, ke);
+                       n_elem * NQ4, r, shear, lame, ke);
     SSX_CHECK_LAUNCH("ssx_element_matrices_p4");
     return SSX_OK;
 }

@@ -120,6 +120,16 @@ class DevicePattern:
         self.weight = to_dev(geom.weight)
         self.max_block_nnz = pat.max_block_nnz(64)
         self.lanes = spmv_lanes(pat.indptr)
+        self.ns = None
+        if self.dim == 3 and self.n_p in (4, 10, 35):
+            from .pattern import node_slots
+            ns = node_slots(pat)
+            self.ns = ns
+            self.ns_ptr = to_dev(ns.ptr)
+            self.ns_elem = to_dev(ns.elem)
+            self.ns_lnode = to_dev(ns.lnode)
+            self.ns_off = to_dev(ns.off)
+        self._ke = None
 
     def new_csr(self, with32: bool = True) -> DeviceCSR:
         dev = self.indptr.device
@@ -139,28 +149,62 @@ class DevicePattern:
         _lib.call("ssx_force_gather", ptr(fe), ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow),
                   self.n_rows, self.nld, ptr(F), stream_ptr())
 
-    def element_matrices_p4(self, DS: torch.Tensor) -> torch.Tensor:
-        """(n_elem, 105, 105) element tangents of a P4 mesh on the FP64 matrix cores."""
-        if getattr(self, "_ke", None) is None:
-            self._ke = torch.empty((self.n_elem, self.nld, self.nld), dtype=torch.float64, device=DS.device)
-        _lib.call("ssx_element_matrices_p4", ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem, ptr(self._ke),
-                  stream_ptr())
+    def _ke_buffer(self, dev) -> torch.Tensor:
+        if self._ke is None:
+            self._ke = torch.empty((self.n_elem, self.nld, self.nld), dtype=torch.float64, device=dev)
         return self._ke
+
+    def element_matrices_p4(self, DS: torch.Tensor, r: float = 0.0, mat=None) -> torch.Tensor:
+        """(n_elem, 105, 105) element tangents of a P4 mesh on the FP64 matrix cores; with
+        mat = (shear, lame) per point, the regularised (1-r) K + r K_el element matrices."""
+        ke = self._ke_buffer(DS.device)
+        sh, la = (None, None) if mat is None else mat
+        _lib.call("ssx_element_matrices_p4", ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem, float(r),
+                  ptr(sh), ptr(la), ptr(ke), stream_ptr())
+        return ke
+
+    def element_matrices_3d(self, DS: torch.Tensor, r: float = 0.0, mat=None) -> torch.Tensor:
+        """(n_elem, nld, nld) element tangents of a P1/P2 tetrahedral mesh (one wavefront per element)."""
+        ke = self._ke_buffer(DS.device)
+        sh, la = (None, None) if mat is None else mat
+        _lib.call("ssx_element_matrices_3d", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
+                  self.n_q, float(r), ptr(sh), ptr(la), ptr(ke), stream_ptr())
+        return ke
 
     def element_matrices_blocks(self, DS: torch.Tensor) -> torch.Tensor:
         """(n_elem, nld, nld) element tangents of a P1/P2 tetrahedral mesh (node-block kernel)."""
-        if getattr(self, "_ke", None) is None:
-            self._ke = torch.empty((self.n_elem, self.nld, self.nld), dtype=torch.float64, device=DS.device)
+        ke = self._ke_buffer(DS.device)
         _lib.call("ssx_element_matrices_blocks", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
-                  self.n_q, ptr(self._ke), stream_ptr())
-        return self._ke
+                  self.n_q, ptr(ke), stream_ptr())
+        return ke
 
-    tangent_method = "rows"   # P1/P2 3D: "rows" (fused row kernel) or "elem" (element blocks + row gather)
+    def gather_nodes(self, ke: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
+                     elastic: torch.Tensor | None = None) -> None:
+        _lib.call("ssx_gather_node_rows", self.dim, self.nld, ptr(ke), ptr(self.ns_ptr), ptr(self.ns_elem),
+                  ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), self.host.n_nodes,
+                  self.ns.max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
+
+    # 3D tetrahedra: "nodes" (element matrices + node gather, the default), "rows" (fused row
+    # kernel) or "elem" (node-block element matrices + row gather)
+    tangent_method = "nodes"
 
     def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
-                elastic: torch.Tensor | None = None) -> None:
-        if self.dim == 3 and (self.n_p == 35 or self.tangent_method == "elem"):
-            ke = self.element_matrices_p4(DS) if self.n_p == 35 else self.element_matrices_blocks(DS)
+                elastic: torch.Tensor | None = None, mat=None) -> None:
+        """Tangent values on the pattern.  K_r = (1-r) K + r K_el either from the assembled elastic
+        values (`elastic`) or folded into the point tangents (`mat` = (shear, lame) per point)."""
+        if mat is not None and self.ns is None:
+            raise ValueError("the K_r fold needs the node-gather path (3D tetrahedra)")
+        if self.ns is not None and (self.n_p == 35 or self.tangent_method == "nodes"):
+            if self.n_p == 35:
+                ke = self.element_matrices_p4(DS, r, mat)
+            else:
+                ke = self.element_matrices_3d(DS, r, mat)
+            self.gather_nodes(ke, out, out32, r, None if mat is not None else elastic)
+            return
+        if mat is not None:
+            raise ValueError("the K_r fold needs the node-gather path")
+        if self.dim == 3 and self.tangent_method == "elem":
+            ke = self.element_matrices_blocks(DS)
             _lib.call("ssx_scatter_element_rows", ptr(ke), self.nld, ptr(self.row_slot_ptr), ptr(self.slot_elem),
                       ptr(self.slot_lrow), ptr(self.slot_pos), ptr(self.indptr), ptr(self.row_free), self.n_rows,
                       float(r), ptr(elastic), ptr(out), ptr(out32), self.max_block_nnz, stream_ptr())

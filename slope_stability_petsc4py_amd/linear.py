@@ -339,15 +339,8 @@ class PMultigridP4:
         self.pat2 = DevicePattern(pat2, geom2)
         dev = self.pat2.indptr.device
         self.t42 = _Transfer(*prows, mesh.q_mask.size, n2, dev)
-        # elastic values of the P2 level (same material points)
-        from .constitutive import constitutive_problem_3d
-        n = geom2.n_int
-        zeros = torch.zeros((n, 6), dtype=torch.float64, device=dev)
-        huge = torch.full((n,), 1e300, dtype=torch.float64, device=dev)
-        _S, DSel = constitutive_problem_3d(zeros, huge, torch.zeros(n, dtype=torch.float64, device=dev), op.shear,
-                                          op.lame, return_tangent=True)
-        self.el2 = torch.empty(pat2.nnz, dtype=torch.float64, device=dev)
-        self.pat2.tangent(DSel, self.el2)
+        # the P2 level shares the P4 points (same rule), so K_r at that level folds the same
+        # per-point shear and lame into its element matrices
         self.A2 = self.pat2.new_csr(with32=True)
         self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac,
                                  coarse_refresh=coarse_refresh)
@@ -359,7 +352,7 @@ class PMultigridP4:
             self.level = _Level(A, self.use32)
         self.level.refresh()
         r = float(getattr(self.op, "last_regularization", self.r))
-        self.pat2.tangent(self.op._DS, self.A2.values, self.A2.values32, r=r, elastic=self.el2)
+        self.pat2.tangent(self.op._DS, self.A2.values, self.A2.values32, r=r, mat=(self.op.shear, self.op.lame))
         self.coarse.setup(self.A2)
         dt = torch.float32 if self.use32 else torch.float64
         dev = A.indptr.device

@@ -165,3 +165,49 @@ def build_owned_pattern(elem: np.ndarray, q_mask: np.ndarray) -> OwnedPattern:
         free_idx=np.flatnonzero(q_mask.reshape(-1, order="F")).astype(np.int64),
         stats={"n_slots": float(n_slots), "nnz": float(nnz)},
     )
+
+
+@dataclass
+class NodeSlots:
+    """Per-node element lists of the node-gather assembly.
+
+    For every node with at least one free component: the elements touching it in element order
+    (the row-slot order of its free rows), the node's local index in each, and the CSR position of
+    every local column relative to the start of the node's rows (int16, -1 for constrained
+    columns).  All free rows of a node hold the same column list, so one offset table serves them
+    all.  Nodes without free components get no slots (their rows are unit rows)."""
+
+    ptr: np.ndarray        # (n_nodes+1,) int32
+    elem: np.ndarray       # (n_slots,) int32
+    lnode: np.ndarray      # (n_slots,) uint8
+    off: np.ndarray        # (n_slots, dim*n_p) int16
+    max_len: int           # longest free row
+
+
+def node_slots(pat: OwnedPattern) -> NodeSlots:
+    dim, n_nodes = pat.dim, pat.n_nodes
+    nld = dim * pat.n_p
+    free = pat.row_free.reshape(n_nodes, dim).astype(bool)
+    has_free = free.any(axis=1)
+    r0 = dim * np.arange(n_nodes, dtype=np.int64) + np.argmax(free, axis=1)   # first free row of each node
+    rsp = pat.row_slot_ptr.astype(np.int64)
+    ip = pat.indptr.astype(np.int64)
+    cnt = np.where(has_free, rsp[r0 + 1] - rsp[r0], 0)
+    ptr = np.zeros(n_nodes + 1, dtype=np.int64)
+    ptr[1:] = np.cumsum(cnt)
+    n = int(ptr[-1])
+    _check_int32("node_slot_count", n)
+    sel = np.repeat(rsp[r0] - ptr[:-1], cnt) + np.arange(n)                    # row-slot index per node slot
+    base = np.repeat(ip[r0], cnt)
+    pos = pat.slot_pos[sel].astype(np.int64)
+    off = np.where(pos >= 0, pos - base[:, None], -1)
+    row_len = np.diff(ip)
+    max_len = int(row_len[pat.row_free.astype(bool)].max()) if pat.row_free.any() else 1
+    if max_len >= 32768:
+        raise OverflowError(f"row length {max_len} exceeds the int16 column offsets")
+    node_len = np.repeat(row_len[r0], cnt)
+    if n and (off.max(axis=1) >= node_len).any():
+        raise AssertionError("node-slot column offset outside its row")
+    return NodeSlots(ptr=ptr.astype(np.int32), elem=pat.slot_elem[sel].astype(np.int32),
+                     lnode=(pat.slot_lrow[sel] // dim).astype(np.uint8),
+                     off=np.ascontiguousarray(off.astype(np.int16).reshape(n, nld)), max_len=max_len)

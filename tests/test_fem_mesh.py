@@ -132,3 +132,32 @@ def test_xyz_order_is_x_major():
     c = m.coord
     keys = np.lexsort((c[2], c[1], c[0]))
     np.testing.assert_array_equal(keys, np.arange(c.shape[1]))
+
+
+def test_node_slots_share_offsets_across_free_rows():
+    """One int16 offset table per (node, element) serves every free row of the node: it equals
+    slot_pos - row start for each of them; slots are in element order."""
+    from slope_stability_petsc4py_amd import pattern
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=3)
+    pat = pattern.build_owned_pattern(m.elem, m.q_mask)
+    ns = pattern.node_slots(pat)
+    dim = pat.dim
+    ip = pat.indptr.astype(np.int64)
+    rsp = pat.row_slot_ptr.astype(np.int64)
+    checked = 0
+    for node in range(pat.n_nodes):
+        a, b = ns.ptr[node], ns.ptr[node + 1]
+        assert np.all(np.diff(ns.elem[a:b]) > 0)
+        for c in range(dim):
+            row = dim * node + c
+            if not pat.row_free[row]:
+                continue
+            s0, s1 = rsp[row], rsp[row + 1]
+            assert s1 - s0 == b - a
+            np.testing.assert_array_equal(pat.slot_elem[s0:s1], ns.elem[a:b])
+            np.testing.assert_array_equal(pat.slot_lrow[s0:s1] // dim, ns.lnode[a:b])
+            pos = pat.slot_pos[s0:s1].astype(np.int64)
+            exp = np.where(pos >= 0, pos - ip[row], -1)
+            np.testing.assert_array_equal(ns.off[a:b], exp)
+            checked += 1
+    assert checked > 0 and ns.max_len == int(np.diff(ip)[pat.row_free.astype(bool)].max())

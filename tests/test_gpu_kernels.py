@@ -173,11 +173,14 @@ def test_tangent_global_fallback_path(gpu, asm3d):
     """The LDS-tiled and the global-memory variants of the tangent kernel agree bitwise."""
     m, geom, pat = asm3d
     dp = DevicePattern(pat, geom)
+    dp.tangent_method = "rows"
     dev = dp.indptr.device
     DS = torch.from_numpy(np.random.default_rng(2).normal(size=(geom.n_int, 36))).to(dev)
     K1 = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
     K2 = torch.empty_like(K1)
     dp.tangent(DS, K1)
+    Ko = oracle.tangent_3d_rows(geom.dphi, DS.cpu().numpy(), geom.weight, pat, geom.n_q)
+    assert _rel(K1.cpu().numpy(), Ko) < 1e-12
     dp.max_block_nnz = -1  # forces the global-memory path
     dp.tangent(DS, K2)
     assert torch.equal(K1, K2)
@@ -335,3 +338,45 @@ def test_p2_element_blocks_path_vs_oracle(gpu, asm3d):
     exp = 0.8 * Ko + 0.2 * el.cpu().numpy()
     exp[pat.constrained_diag_positions] = 0.8 + 0.2 * el.cpu().numpy()[pat.constrained_diag_positions]
     assert _rel(K.cpu().numpy(), exp) < 1e-12
+
+
+@pytest.mark.parametrize("elem_type", ["P1", "P2", "P4"])
+def test_node_gather_kr_fold_vs_oracle(gpu, elem_type):
+    """Element matrices with the regularisation folded into the point tangents, gathered per node:
+    K_r == (1-r) K(DS) + r K(D_el) of the oracle rows kernel, FP32 copy exact, bitwise repeatable."""
+    print(f"case: node gather K_r fold {elem_type}", flush=True)
+    nz = 1 if elem_type == "P4" else 3
+    m = mesh.slope_3d(h=5.0 if elem_type == "P4" else 2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0,
+                      z_len=10.0, nz=nz, elem_type=elem_type)
+    geom = fem.geometry(m.coord, m.elem, elem_type)
+    pat = pattern.build_owned_pattern(m.elem, m.q_mask)
+    dp = DevicePattern(pat, geom)
+    assert dp.tangent_method == "nodes" and dp.ns is not None
+    dev = dp.indptr.device
+    rng = np.random.default_rng(41)
+    A = rng.normal(size=(geom.n_int, 6, 6))
+    DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
+    G = rng.uniform(1e3, 2e4, geom.n_int)
+    lam = rng.uniform(1e3, 3e4, geom.n_int)
+    vol = np.array([1, 1, 1, 0, 0, 0.0])
+    idn = np.diag([1, 1, 1, 0.5, 0.5, 0.5])
+    Del = (lam[:, None, None] * np.outer(vol, vol)[None] + 2 * G[:, None, None] * idn[None]).reshape(-1, 36)
+    r = 0.3
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    K32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev)
+    dp.tangent(t(DS), K, K32, r=r, mat=(t(G), t(lam)))
+    Kt = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+    Kel = oracle.tangent_3d_rows(geom.dphi, Del, geom.weight, pat, geom.n_q)
+    exp = (1 - r) * Kt + r * Kel
+    exp[pat.constrained_diag_positions] = 1.0
+    assert _rel(K.cpu().numpy(), exp) < 1e-12
+    np.testing.assert_array_equal(K32.cpu().numpy(), K.cpu().numpy().astype(np.float32))
+    K2 = torch.empty_like(K)
+    dp.tangent(t(DS), K2, r=r, mat=(t(G), t(lam)))
+    assert torch.equal(K, K2)
+    # unregularised node gather == oracle rows kernel
+    dp.tangent(t(DS), K2)
+    exp0 = Kt.copy()
+    exp0[pat.constrained_diag_positions] = 1.0
+    assert _rel(K2.cpu().numpy(), exp0) < 1e-12
