@@ -142,6 +142,11 @@ int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *
 int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
                             double r, const double *shear, const double *lame, double *ke, void *stream);
 
+/* Tuning knob of ssx_element_matrices_3d (no reference counterpart): variant 0 = one wavefront per
+ * element with direct loads, 1 = persistent pipelined wavefronts with D from LDS (default),
+ * 2 = pipelined with D by scalar loads; waves_per_cu sizes the persistent grid. */
+int ssx_set_element_kernel(int variant, int waves_per_cu);
+
 /* Node gather of element matrices into the fixed CSR pattern: one wavefront per mesh node sums the
  * node's dim rows over its elements (node slots ns_ptr/ns_elem/ns_lnode, element order) at the
  * int16 column offsets ns_off (n_slots x nld, relative to the row start, -1 = constrained column),

@@ -44,6 +44,7 @@ _SIGS: dict[str, list] = {
     "ssx_galerkin_dense": [I64, P, P, P, P, P, P, I64, P, P],
     "ssx_element_matrices_p4": [P, P, P, I64, F64, P, P, P, P],
     "ssx_element_matrices_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P],
+    "ssx_set_element_kernel": [I32, I32],
     "ssx_gather_node_rows": [I32, I32, P, P, P, P, P, P, P, I64, I32, F64, P, P, P, P],
     "ssx_scatter_element_rows": [P, I32, P, P, P, P, P, P, I64, F64, P, P, P, I64, P],
     "ssx_gemm_bf16": [I64, I64, I64, P, I64, I64, P, I64, I64, P, I64, P, I32, P, F32, P],

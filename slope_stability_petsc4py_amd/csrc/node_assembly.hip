@@ -115,6 +115,186 @@ __global__ void __launch_bounds__(64) k_ke_3d(const double *__restrict__ dphi, c
 }
 
 // ------------------------------------------------------------------------------------------
+// Phase 1, pipelined form for P2 (NP = 10, NQ = 11) and P1 (NP = 4, NQ = 1): persistent
+// wavefronts walk elements e, e + grid, ...; while one element is computed from LDS, the next
+// element's point data (gradients, D, w and the material) is already in flight into registers
+// and is written to the other LDS buffer afterwards, so the global-memory latency is paid once
+// per element and overlapped with the previous element's arithmetic.
+//
+// LDS per buffer: gradients sg[q][c][h][6] (the two column halves of NH <= 5 nodes each padded to
+// 6, so a lane reads its columns as aligned pairs), D transposed to rows sd[q][s][m] = D[s + 6 m]
+// (a lane reads the three rows of D its B row touches), and w, shear, lame per point.
+// DSMEM: D is read with scalar loads from global memory instead (the staging of D into LDS then
+// only brings the element's D into the L2 before those scalar loads).
+// ------------------------------------------------------------------------------------------
+template <int NP, int NQ>
+struct KePipe {
+    static constexpr int NLD = 3 * NP;
+    static constexpr int NH = NP / 2;
+    static constexpr int HP = 6;                   // padded half width
+    static constexpr int G = NQ * 3 * 2 * HP;      // gradient words
+    static constexpr int D = NQ * 36;              // D words
+    static constexpr int M = NQ * 3;               // w, shear, lame
+    static constexpr int WORDS = G + D + M;
+    static constexpr int NGR = (NQ * 3 * NP + 63) / 64;   // gradient values per lane in flight
+    static constexpr int NDR = (D + 63) / 64;
+    static constexpr int NMR = (M + 63) / 64;
+};
+
+template <int NP, int NQ, bool FOLD, bool DSMEM>
+__global__ void __launch_bounds__(64) k_ke_3d_pipe(const double *__restrict__ dphi, const double *__restrict__ DS,
+                                                   const double *__restrict__ w, int64_t n_elem, double r,
+                                                   const double *__restrict__ shear, const double *__restrict__ lame,
+                                                   double *__restrict__ ke) {
+    using P = KePipe<NP, NQ>;
+    constexpr int NLD = P::NLD, NH = P::NH, HP = P::HP;
+    __shared__ double lds[2][P::WORDS];
+    const int lane = threadIdx.x;
+    const int h = lane >> 5;
+    const int a_raw = lane & 31;
+    const bool valid = a_raw < NLD;
+    const int a = valid ? a_raw : NLD - 1;
+    const int ni = a / 3, ci = a - 3 * ni;
+    const int64_t n_int = n_elem * NQ;
+    // rows of D touched by row a of B^T (component ci): s0 = ci, s1 = 3 + (ci == 2), s2 = 5 - (ci == 1)
+    const int s0 = ci, s1 = 3 + (ci == 2 ? 1 : 0), s2 = 5 - (ci == 1 ? 1 : 0);
+    const int hi = ni / NH, ki = ni - NH * (ni / NH);   // node ni's half and slot
+
+    double gr[P::NGR], dr[P::NDR], mr[P::NMR][3];
+    // global -> registers for element e (all loads in bounds; lanes past the data read index 0)
+    auto fetch = [&](int64_t e) {
+#pragma unroll
+        for (int k = 0; k < P::NGR; ++k) {
+            const int t = lane + 64 * k;                       // t = (c * NQ + q) * NP + node
+            const int tc = t < NQ * 3 * NP ? t : 0;
+            const int c = tc / (NQ * NP), rem = tc - c * (NQ * NP);
+            gr[k] = dphi[((int64_t)c * n_int + e * NQ) * NP + rem];
+        }
+#pragma unroll
+        for (int k = 0; k < P::NDR; ++k) {
+            const int t = lane + 64 * k;
+            dr[k] = DS[e * P::D + (t < P::D ? t : 0)];
+        }
+#pragma unroll
+        for (int k = 0; k < P::NMR; ++k) {
+            // t = kind * NQ + q: w, shear and lame are each read at an in-bounds index (no per-lane
+            // choice between base pointers); the value is selected when staged, so nothing here
+            // waits for the loads
+            const int t = lane + 64 * k;
+            const int tc = t < P::M ? t : 0;
+            const int kind = tc / NQ, q = tc - kind * NQ;
+            mr[k][0] = w[e * NQ + q];
+            if constexpr (FOLD) {
+                mr[k][1] = shear[e * NQ + q];
+                mr[k][2] = lame[e * NQ + q];
+            }
+        }
+    };
+    // registers -> LDS buffer b
+    auto stage = [&](int b) {
+        double *L = lds[b];
+#pragma unroll
+        for (int k = 0; k < P::NGR; ++k) {
+            const int t = lane + 64 * k;
+            if (t < NQ * 3 * NP) {
+                const int c = t / (NQ * NP), rem = t - c * (NQ * NP);
+                const int q = rem / NP, node = rem - q * NP;
+                const int hh = node / NH, kk = node - NH * (node / NH);
+                L[((q * 3 + c) * 2 + hh) * HP + kk] = gr[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < P::NDR; ++k) {
+            const int t = lane + 64 * k;                       // t = q*36 + s + 6 m  -> row-major (s, m)
+            if (t < P::D) {
+                const int q = t / 36, rem = t - 36 * q;
+                const int m = rem / 6, s = rem - 6 * m;
+                L[P::G + q * 36 + s * 6 + m] = dr[k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < P::NMR; ++k) {
+            const int t = lane + 64 * k;
+            const int kind = (t < P::M ? t : 0) / NQ;
+            double v = kind == 0 ? mr[k][0] : 0.0;
+            if constexpr (FOLD) v += (kind == 1 ? mr[k][1] : 0.0) + (kind == 2 ? mr[k][2] : 0.0);
+            if (t < P::M) L[P::G + P::D + t] = v;
+        }
+    };
+
+    int64_t e = blockIdx.x;
+    if (e >= n_elem) return;   // whole wavefront (one wavefront per workgroup)
+    fetch(e);
+    stage(0);
+    __syncthreads();
+    const double om = FOLD ? 1.0 - r : 1.0;
+    int buf = 0;
+    for (; e < n_elem; e += gridDim.x) {
+        const int64_t en = e + gridDim.x;
+        if (en < n_elem) fetch(en);
+        const double *L = lds[buf];
+        double acc[3 * NH];
+#pragma unroll
+        for (int k = 0; k < 3 * NH; ++k) acc[k] = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < NQ; ++q) {
+            const double *gq = L + q * 3 * 2 * HP;
+            const double xi = gq[(0 * 2 + hi) * HP + ki], yi = gq[(1 * 2 + hi) * HP + ki], zi = gq[(2 * 2 + hi) * HP + ki];
+            const double wg = L[P::G + P::D + q];
+            double t[6];
+            if constexpr (DSMEM) {
+                const double *Dg = DS + (e * NQ + q) * 36;   // column-major D[s + 6 m]
+                double b[6];
+                brow(ci, xi, yi, zi, b);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    double v = b[0] * Dg[6 * m];
+#pragma unroll
+                    for (int ss = 1; ss < 6; ++ss) v += b[ss] * Dg[ss + 6 * m];
+                    t[m] = v;
+                }
+            } else {
+                // B row a has coefficients (c0, c1, c2) on strain rows (s0, s1, s2)
+                const double c0 = ci == 0 ? xi : (ci == 1 ? yi : zi);
+                const double c1 = ci == 1 ? xi : yi;
+                const double c2 = ci == 2 ? xi : zi;
+                const double *D0 = L + P::G + q * 36 + s0 * 6;
+                const double *D1 = L + P::G + q * 36 + s1 * 6;
+                const double *D2 = L + P::G + q * 36 + s2 * 6;
+#pragma unroll
+                for (int m = 0; m < 6; ++m) t[m] = c0 * D0[m] + c1 * D1[m] + c2 * D2[m];
+            }
+            if constexpr (FOLD) {
+                const double Gq = L[P::G + P::D + NQ + q], Lq = L[P::G + P::D + 2 * NQ + q];
+                double b[6];
+                brow(ci, xi, yi, zi, b);
+                const double tr = Lq * (b[0] + b[1] + b[2]);
+#pragma unroll
+                for (int m = 0; m < 6; ++m) t[m] = om * t[m] + r * (m < 3 ? tr + 2.0 * Gq * b[m] : Gq * b[m]);
+            }
+#pragma unroll
+            for (int m = 0; m < 6; ++m) t[m] *= wg;
+            const double *gx = gq + (0 * 2 + h) * HP, *gy = gq + (1 * 2 + h) * HP, *gz = gq + (2 * 2 + h) * HP;
+#pragma unroll
+            for (int k = 0; k < NH; ++k) {
+                const double xj = gx[k], yj = gy[k], zj = gz[k];
+                acc[3 * k + 0] += t[0] * xj + t[3] * yj + t[5] * zj;
+                acc[3 * k + 1] += t[3] * xj + t[1] * yj + t[4] * zj;
+                acc[3 * k + 2] += t[5] * xj + t[4] * yj + t[2] * zj;
+            }
+        }
+        if (valid) {
+            double *o = ke + (e * NLD + a) * NLD + 3 * NH * h;
+#pragma unroll
+            for (int k = 0; k < 3 * NH; ++k) o[k] = acc[k];
+        }
+        if (en < n_elem) stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
 // Phase 2: node gather.  One wavefront per node; LDS holds the node's DIM rows (free rows share
 // one length L) plus a dummy word that absorbs the updates of constrained columns and rows.  Each
 // element contributes its DIM contiguous rows DIM*li .. DIM*li + DIM-1 (DIM*NLD values, read
@@ -246,7 +426,21 @@ int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_ele
 
 }  // namespace
 
+// Element-matrix kernel selection (tuning knob): 0 = one wavefront per element with direct
+// loads, 1 = pipelined with D from LDS, 2 = pipelined with D by scalar loads; waves per CU of the
+// persistent grid.
+static int ssx_ke_variant = 1;
+static int ssx_ke_waves_per_cu = 16;
+
 extern "C" {
+
+int ssx_set_element_kernel(int variant, int waves_per_cu) {
+    SSX_REQUIRE(variant >= 0 && variant <= 2 && waves_per_cu > 0 && waves_per_cu <= 64,
+                "ssx_set_element_kernel: variant 0..2, waves_per_cu 1..64");
+    ssx_ke_variant = variant;
+    ssx_ke_waves_per_cu = waves_per_cu;
+    return SSX_OK;
+}
 
 int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
                             double r, const double *shear, const double *lame, double *ke, void *stream) {
@@ -256,6 +450,31 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
     SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_matrices_3d: shear and lame go together");
     SSX_REQUIRE(n_elem <= 0x7fffffff, "ssx_element_matrices_3d: too many elements for one launch");
     hipStream_t st = ssx::as_stream(stream);
+    const int variant = ssx_ke_variant;
+    if (variant > 0 && ((n_p == 10 && n_q == 11) || (n_p == 4 && n_q == 1))) {
+        static int n_cu = 0;
+        if (n_cu == 0) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
+                n_cu = 256;
+        }
+        const int64_t want = (int64_t)n_cu * ssx_ke_waves_per_cu;
+        const unsigned grid = (unsigned)(n_elem < want ? n_elem : want);
+        const bool fold = shear != nullptr;
+#define PIPE(NP_, NQ_, F_, S_) \
+    hipLaunchKernelGGL((k_ke_3d_pipe<NP_, NQ_, F_, S_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, ke)
+        if (n_p == 10) {
+            if (variant == 1) { if (fold) PIPE(10, 11, true, false); else PIPE(10, 11, false, false); }
+            else { if (fold) PIPE(10, 11, true, true); else PIPE(10, 11, false, true); }
+        } else {
+            if (variant == 1) { if (fold) PIPE(4, 1, true, false); else PIPE(4, 1, false, false); }
+            else { if (fold) PIPE(4, 1, true, true); else PIPE(4, 1, false, true); }
+        }
+#undef PIPE
+        SSX_CHECK_LAUNCH("ssx_element_matrices_3d");
+        return SSX_OK;
+    }
     if (n_p == 10)
         hipLaunchKernelGGL((k_ke_3d<10>), dim3((unsigned)n_elem), dim3(64), 0, st, dphi, DS, w, n_elem, n_q, r, shear,
                            lame, ke);
