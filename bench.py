@@ -154,6 +154,14 @@ def run_gpu(args):
             b.record()
             ev[name].append((a, b))
     torch.cuda.synchronize()
+    if args.copy_probe:
+        # counter calibration: a plain device copy of a known size (256 MiB read + 256 MiB written)
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     ab = algorithmic_bytes(op, fold=mat is not None)
     kern = {}
     for name, lst in ev.items():
@@ -331,6 +339,8 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run a 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows", "elem"],
                     help="P2 tangent path: element matrices + node gather (default), fused row kernel, or "
                          "element blocks + row gather")

@@ -376,3 +376,25 @@ def brute_pattern(elem: np.ndarray, q_mask: np.ndarray):
         chunks.append(a)
         indptr[r + 1] = indptr[r] + a.size
     return indptr, np.concatenate(chunks)
+
+
+def a_orthogonalize(B: np.ndarray, A) -> tuple[np.ndarray, np.ndarray]:
+    """A-orthonormalisation of the rows of B newest-first with the drop rule of
+    linear/orthogonalize.py:124-170: v <- v - sum_j s_j (w_j . A v0) w_j with A v0 taken before the
+    projection, kept when |v . A v0| > 1e-3, scaled by 1/sqrt|v . A v0|, sign s = sign(v . A v0).
+    Returns (W oldest-first, signs).  Pinned by the closed-form checks in its tests (W^T A W =
+    diag(s) on a well-conditioned basis; exact dependence is dropped)."""
+    k = B.shape[0]
+    W, S = [], []
+    for i in range(k):
+        v = np.array(B[k - 1 - i], dtype=np.float64)
+        av = A @ v
+        for w, s in zip(W, S):
+            v = v - s * float(w @ av) * w
+        nv = float(v @ av)
+        if abs(nv) > 1e-3:
+            W.append(v / np.sqrt(abs(nv)))
+            S.append(1.0 if nv > 0 else -1.0)
+    if not W:
+        return np.zeros((0, B.shape[1])), np.zeros(0)
+    return np.array(W[::-1]), np.array(S[::-1])

@@ -141,14 +141,13 @@ struct KePipe {
     static constexpr int NMR = (M + 63) / 64;
 };
 
-template <int NP, int NQ, bool FOLD, bool DSMEM>
-__global__ void __launch_bounds__(64) k_ke_3d_pipe(const double *__restrict__ dphi, const double *__restrict__ DS,
-                                                   const double *__restrict__ w, int64_t n_elem, double r,
-                                                   const double *__restrict__ shear, const double *__restrict__ lame,
-                                                   double *__restrict__ ke) {
+template <int NP, int NQ, bool FOLD, bool DSMEM, int NBUF>
+__device__ __forceinline__ void ke_pipe_body(const double *__restrict__ dphi, const double *__restrict__ DS,
+                                             const double *__restrict__ w, int64_t n_elem, double r,
+                                             const double *__restrict__ shear, const double *__restrict__ lame,
+                                             double *__restrict__ ke, double (*lds)[KePipe<NP, NQ>::WORDS]) {
     using P = KePipe<NP, NQ>;
     constexpr int NLD = P::NLD, NH = P::NH, HP = P::HP;
-    __shared__ double lds[2][P::WORDS];
     const int lane = threadIdx.x;
     const int h = lane >> 5;
     const int a_raw = lane & 31;
@@ -288,10 +287,36 @@ __global__ void __launch_bounds__(64) k_ke_3d_pipe(const double *__restrict__ dp
 #pragma unroll
             for (int k = 0; k < 3 * NH; ++k) o[k] = acc[k];
         }
-        if (en < n_elem) stage(buf ^ 1);
-        __syncthreads();
-        buf ^= 1;
+        if constexpr (NBUF == 2) {
+            if (en < n_elem) stage(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        } else {
+            __syncthreads();   // every lane is done reading the buffer
+            if (en < n_elem) stage(0);
+            __syncthreads();
+        }
     }
+}
+
+// Double-buffered LDS (the next element is staged while nothing waits for it).
+template <int NP, int NQ, bool FOLD, bool DSMEM>
+__global__ void __launch_bounds__(64) k_ke_3d_pipe(const double *__restrict__ dphi, const double *__restrict__ DS,
+                                                   const double *__restrict__ w, int64_t n_elem, double r,
+                                                   const double *__restrict__ shear, const double *__restrict__ lame,
+                                                   double *__restrict__ ke) {
+    __shared__ double lds[2][KePipe<NP, NQ>::WORDS];
+    ke_pipe_body<NP, NQ, FOLD, DSMEM, 2>(dphi, DS, w, n_elem, r, shear, lame, ke, lds);
+}
+
+// Single LDS buffer and at least four wavefronts per SIMD (register budget 128).
+template <int NP, int NQ, bool FOLD, bool DSMEM>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
+k_ke_3d_pipe1(const double *__restrict__ dphi, const double *__restrict__ DS, const double *__restrict__ w,
+              int64_t n_elem, double r, const double *__restrict__ shear, const double *__restrict__ lame,
+              double *__restrict__ ke) {
+    __shared__ double lds[1][KePipe<NP, NQ>::WORDS];
+    ke_pipe_body<NP, NQ, FOLD, DSMEM, 1>(dphi, DS, w, n_elem, r, shear, lame, ke, lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -427,16 +452,16 @@ int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_ele
 }  // namespace
 
 // Element-matrix kernel selection (tuning knob): 0 = one wavefront per element with direct
-// loads, 1 = pipelined with D from LDS, 2 = pipelined with D by scalar loads; waves per CU of the
-// persistent grid.
+// loads, 1 = pipelined with D from LDS, 2 = pipelined with D by scalar loads, 3 = pipelined with a
+// single LDS buffer and a 128-register budget; waves per CU of the persistent grid.
 static int ssx_ke_variant = 1;
 static int ssx_ke_waves_per_cu = 16;
 
 extern "C" {
 
 int ssx_set_element_kernel(int variant, int waves_per_cu) {
-    SSX_REQUIRE(variant >= 0 && variant <= 2 && waves_per_cu > 0 && waves_per_cu <= 64,
-                "ssx_set_element_kernel: variant 0..2, waves_per_cu 1..64");
+    SSX_REQUIRE(variant >= 0 && variant <= 3 && waves_per_cu > 0 && waves_per_cu <= 64,
+                "ssx_set_element_kernel: variant 0..3, waves_per_cu 1..64");
     ssx_ke_variant = variant;
     ssx_ke_waves_per_cu = waves_per_cu;
     return SSX_OK;
@@ -462,14 +487,16 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
         const int64_t want = (int64_t)n_cu * ssx_ke_waves_per_cu;
         const unsigned grid = (unsigned)(n_elem < want ? n_elem : want);
         const bool fold = shear != nullptr;
-#define PIPE(NP_, NQ_, F_, S_) \
-    hipLaunchKernelGGL((k_ke_3d_pipe<NP_, NQ_, F_, S_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, ke)
+#define PIPE(K_, NP_, NQ_, F_, S_) \
+    hipLaunchKernelGGL((K_<NP_, NQ_, F_, S_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, ke)
         if (n_p == 10) {
-            if (variant == 1) { if (fold) PIPE(10, 11, true, false); else PIPE(10, 11, false, false); }
-            else { if (fold) PIPE(10, 11, true, true); else PIPE(10, 11, false, true); }
+            if (variant == 1) { if (fold) PIPE(k_ke_3d_pipe, 10, 11, true, false); else PIPE(k_ke_3d_pipe, 10, 11, false, false); }
+            else if (variant == 2) { if (fold) PIPE(k_ke_3d_pipe, 10, 11, true, true); else PIPE(k_ke_3d_pipe, 10, 11, false, true); }
+            else { if (fold) PIPE(k_ke_3d_pipe1, 10, 11, true, false); else PIPE(k_ke_3d_pipe1, 10, 11, false, false); }
         } else {
-            if (variant == 1) { if (fold) PIPE(4, 1, true, false); else PIPE(4, 1, false, false); }
-            else { if (fold) PIPE(4, 1, true, true); else PIPE(4, 1, false, true); }
+            if (variant == 1) { if (fold) PIPE(k_ke_3d_pipe, 4, 1, true, false); else PIPE(k_ke_3d_pipe, 4, 1, false, false); }
+            else if (variant == 2) { if (fold) PIPE(k_ke_3d_pipe, 4, 1, true, true); else PIPE(k_ke_3d_pipe, 4, 1, false, true); }
+            else { if (fold) PIPE(k_ke_3d_pipe1, 4, 1, true, false); else PIPE(k_ke_3d_pipe1, 4, 1, false, false); }
         }
 #undef PIPE
         SSX_CHECK_LAUNCH("ssx_element_matrices_3d");

@@ -325,7 +325,7 @@ class PMultigridP4:
     (Jacobi) smoothing on the P4 and P2 levels, FP32 or FP64."""
 
     def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 1):
+                 coarse_refresh: int = 0):
         from .device import DevicePattern
         from .fem import geometry, quadrature_3d
         from .pattern import build_owned_pattern
@@ -344,6 +344,8 @@ class PMultigridP4:
         self.A2 = self.pat2.new_csr(with32=True)
         self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac,
                                  coarse_refresh=coarse_refresh)
+        self.note_iterations = self.coarse.note_iterations
+        self.timings = self.coarse.timings
         self.level = None
         self.r = 1e-4
 
@@ -394,11 +396,21 @@ class PMultigrid:
     only costs iterations, never correctness)."""
 
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 1):
+                 coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 15):
         self.use32 = use32
         self.degree = degree
         self.lo_frac = lo_frac
-        self.coarse_refresh = max(1, int(coarse_refresh))
+        # coarse_refresh N >= 1: rebuild the coarse operator every N setups; 0: rebuild only when a
+        # solve needed more than `refresh_iterations` FGMRES iterations (the lagged coarse
+        # correction has become too weak)
+        self.coarse_refresh = max(0, int(coarse_refresh))
+        self.refresh_iterations = int(refresh_iterations)
+        self._stale = False
+        # precision of the coarse factorisation: FP64, or FP32 (falls back to FP64 when the FP32
+        # Cholesky meets a non-positive pivot); default follows the level precision
+        self.coarse_precision = coarse_precision or ("fp32" if use32 else "fp64")
+        self.timings = {"level_refresh": 0.0, "galerkin": 0.0, "factor": 0.0, "inverse": 0.0, "coarse_builds": 0,
+                        "coarse_fp64_fallbacks": 0}
         self._setups = 0
         p_ptr, p_idx, p_w, n_c, c_free = p2_to_p1_prolongation(elem, q_mask)
         self.n_c = n_c
@@ -435,29 +447,59 @@ class PMultigrid:
             self._to_device(dev, A.n_rows)
         if self.level is None or self.level.A is not A:
             self.level = _Level(A, self.use32)
+        t0 = perf_counter()
         self.level.refresh()
-        if self.Ainv is None or self._setups % self.coarse_refresh == 0:
+        self.timings["level_refresh"] += perf_counter() - t0
+        due = self.coarse_refresh > 0 and self._setups % self.coarse_refresh == 0
+        if self.Ainv is None or due or self._stale:
             self._coarse(A, dev)
+            self._stale = False
         self._setups += 1
+
+    def note_iterations(self, its: int) -> None:
+        """Called by the Krylov solver after each solve with its iteration count."""
+        if self.coarse_refresh == 0 and its > self.refresh_iterations:
+            self._stale = True
 
     def _coarse(self, A: DeviceCSR, dev) -> None:
         # coarse operator (FP64 accumulate), identity on constrained coarse dofs; the Cholesky
         # factorisation reads the lower triangle only
+        tm = self.timings
+        t0 = perf_counter()
         self.Ainv = None
         Ac = torch.zeros((self.n_c, self.n_c), dtype=torch.float64, device=dev)
         _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.p_ptr),
                   ptr(self.p_idx), ptr(self.p_w), self.n_c, ptr(Ac), _st())
         Ac[self.cfix, self.cfix] = 1.0
-        L, info = torch.linalg.cholesky_ex(Ac)
-        if int(info.item()) != 0:
-            # indefinite coarse operator (strong softening): shifted factorisation
-            shift = 1e-8 * float(Ac.diagonal().abs().max().item())
-            Ac.diagonal().add_(shift)
-            L = torch.linalg.cholesky(Ac)
+        torch.cuda.synchronize()
+        t1 = perf_counter()
+        tm["galerkin"] += t1 - t0
+        L = None
+        if self.coarse_precision == "fp32":
+            L32, info = torch.linalg.cholesky_ex(Ac.to(torch.float32))
+            if int(info.item()) == 0:
+                L = L32
+            else:
+                tm["coarse_fp64_fallbacks"] += 1
+            del L32
+        if L is None:
+            L, info = torch.linalg.cholesky_ex(Ac)
+            if int(info.item()) != 0:
+                # indefinite coarse operator (strong softening): shifted factorisation
+                shift = 1e-8 * float(Ac.diagonal().abs().max().item())
+                Ac.diagonal().add_(shift)
+                L = torch.linalg.cholesky(Ac)
         del Ac
+        torch.cuda.synchronize()
+        t2 = perf_counter()
+        tm["factor"] += t2 - t1
         Ainv = torch.cholesky_inverse(L)
         del L
-        self.Ainv = Ainv.to(torch.float32) if self.use32 else Ainv
+        self.Ainv = Ainv.to(torch.float32) if self.use32 else Ainv.to(torch.float64)
+        del Ainv
+        torch.cuda.synchronize()
+        tm["inverse"] += perf_counter() - t2
+        tm["coarse_builds"] += 1
 
     def cycle(self, b: torch.Tensor, x: torch.Tensor) -> None:
         """One V-cycle on A x = b in the level precision (x overwritten)."""
@@ -569,7 +611,11 @@ class DeflatedFGMRES:
 
     def A_orthogonalize(self, A: DeviceCSR):
         """A-orthonormalise the basis newest-first, dropping columns with |v^T A v| <= eps
-        (linear/orthogonalize.py:49-170)."""
+        (linear/orthogonalize.py:49-170).
+
+        Runs without host round trips per column: the coefficients, the A-norm test and the
+        scaling stay on the device; a dropped column becomes a zero row with sign 0 (so it takes
+        no part in later projections), and the kept rows are compacted after one host read."""
         self._A = A
         B = self.deflation_basis
         if B is None or B.shape[0] == 0:
@@ -581,28 +627,29 @@ class DeflatedFGMRES:
         Wbuf = torch.empty((kmax, n), dtype=torch.float64, device=dev)
         av = torch.empty(n, dtype=torch.float64, device=dev)
         coeff = torch.empty(kmax, dtype=torch.float64, device=dev)
+        sgn = torch.zeros(kmax, dtype=torch.float64, device=dev)
+        nv = torch.empty(kmax, dtype=torch.float64, device=dev)
         scr = self._scr if self._scr is not None else _Scratch(dev)
-        signs: list[float] = []
-        cnt = 0
-        for idx in range(kmax - 1, -1, -1):
-            v = B[idx].clone()
+        for i in range(kmax):
+            v = Wbuf[i]
+            v.copy_(B[kmax - 1 - i])
             spmv(A, v, av)
-            if cnt:
-                _lib.call("ssx_dot_cols", n, cnt, ptr(Wbuf), n, ptr(av), ptr(scr.cols_for(cnt)), ptr(coeff), _st())
-                c = coeff[:cnt].cpu().numpy() * np.asarray(signs)
-                coeff[:cnt].copy_(torch.from_numpy(c))
-                _lib.call("ssx_gemv_cols", n, cnt, ptr(Wbuf), n, ptr(coeff), -1.0, ptr(v), _st())
-            nv = dot(v, av, scr)
-            if abs(nv) > self.tolerance_deflation_basis:
-                axpby(1.0 / np.sqrt(abs(nv)), v, 0.0, Wbuf[cnt])
-                signs.append(1.0 if nv > 0 else -1.0)
-                cnt += 1
-        if cnt == 0:
+            if i:
+                _lib.call("ssx_dot_cols", n, i, ptr(Wbuf), n, ptr(av), ptr(scr.cols_for(i)), ptr(coeff), _st())
+                coeff[:i].mul_(sgn[:i])
+                _lib.call("ssx_gemv_cols", n, i, ptr(Wbuf), n, ptr(coeff), -1.0, ptr(v), _st())
+            _lib.call("ssx_dot", n, ptr(v), ptr(av), ptr(scr.part), ptr(nv[i:]), _st())
+            a = nv[i].abs()
+            keep = a > self.tolerance_deflation_basis
+            v.mul_(torch.where(keep, torch.rsqrt(torch.where(keep, a, torch.ones_like(a))), torch.zeros_like(a)))
+            sgn[i] = torch.where(keep, torch.sign(nv[i]), torch.zeros_like(a))
+        kept = torch.nonzero(sgn != 0).flatten().cpu().numpy()   # the one host read
+        if kept.size == 0:
             self._W = None
             self._AW = None
             return
-        Ws = [Wbuf[i] for i in range(cnt)]
-        W = torch.stack(Ws[::-1]).contiguous()
+        order = torch.from_numpy(kept[::-1].copy()).to(dev)     # oldest first
+        W = Wbuf.index_select(0, order).contiguous()
         AW = torch.empty_like(W)
         for j in range(W.shape[0]):
             spmv(A, W[j], AW[j])
@@ -663,8 +710,11 @@ class DeflatedFGMRES:
                 _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf2), _st())
                 _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf2), -1.0, ptr(u), _st())
                 hbuf[: j + 1] += hbuf2[: j + 1]
-            hn = np.sqrt(max(dot(u, u, scr), 0.0))
-            H[: j + 1, j] = hbuf[: j + 1].cpu().numpy()
+            # |u|^2 lands in hbuf[j+1]: the Hessenberg column and the norm come back in one read
+            _lib.call("ssx_dot", n, ptr(u), ptr(u), ptr(scr.part), ptr(hbuf[j + 1:]), _st())
+            col = hbuf[: j + 2].cpu().numpy()
+            hn = float(np.sqrt(max(col[j + 1], 0.0)))
+            H[: j + 1, j] = col[: j + 1]
             H[j + 1, j] = hn
             t_orth += perf_counter() - to
             iters = j + 1
@@ -684,4 +734,7 @@ class DeflatedFGMRES:
             _lib.call("ssx_gemv_cols", n, iters, ptr(Z), n, ptr(yt), 1.0, ptr(x), _st())
         self.last_residual_history = np.array(hist)
         self.iteration_collector.store(iters, perf_counter() - t0, t_pc, t_orth)
+        note = getattr(self.pc, "note_iterations", None)
+        if note is not None:
+            note(iters)
         return x

@@ -26,8 +26,11 @@ def main(argv=None):
     ap.add_argument("--h", type=float, default=None)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--step-max", type=int, default=None)
-    ap.add_argument("--coarse-refresh", type=int, default=1,
-                    help="rebuild the multigrid coarse operator every N preconditioner setups")
+    ap.add_argument("--coarse-refresh", type=int, default=0,
+                    help="rebuild the multigrid coarse operator every N preconditioner setups (0: when a solve "
+                         "needs more than 15 iterations)")
+    ap.add_argument("--coarse-precision", default=None, choices=["fp32", "fp64"],
+                    help="coarse factorisation precision (default: the preconditioner precision)")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
@@ -39,7 +42,8 @@ def main(argv=None):
         case = cases.slope3d(h=a.h or 2.0, nz=a.nz or 10, hetero=a.case.endswith("hetero"))
     if a.step_max:
         case.params.step_max = a.step_max
-    s = cases.make_solver(case, precision=a.precision, pc=a.pc, coarse_refresh=a.coarse_refresh)
+    s = cases.make_solver(case, precision=a.precision, pc=a.pc, coarse_refresh=a.coarse_refresh,
+                          coarse_precision=a.coarse_precision)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
 
@@ -57,6 +61,7 @@ def main(argv=None):
                "linear_iterations": res.linear_iterations, "attempts": res.attempts,
                "n_dof": int(s.op.pattern.n_rows), "n_elem": int(case.mesh.n_elem),
                "coarse_refresh": a.coarse_refresh, "linear_solver": s.linear.iteration_collector.summary(),
+               "pc_timings": getattr(s.linear.pc, "timings", None),
                "timings": s.op.get_total_time()}
     else:
         res = cases.run_ll(s, progress=prog)

@@ -205,3 +205,25 @@ def test_learned_predictor_exact_guess_is_used():
         1.0, 0.5, 1e-5, 1e-3, 8, 1e9, 50, 10, 1e-9, 1e-4, f, b, s,
         free_mask=torch.ones(n, dtype=torch.float64), dim=3, predictor=exact)
     assert any(r["predictor"] == "learned" for r in res.step_info)
+
+
+def test_oracle_a_orthogonalize_pins():
+    """The A-orthonormalisation oracle: W A W^T = diag(signs) on the kept rows, an exactly
+    dependent row is dropped (newest-first processing keeps the newest), and an indefinite A gives
+    a negative sign."""
+    import oracle
+    rng = np.random.default_rng(0)
+    n = 30
+    M = rng.normal(size=(n, n))
+    A = M @ M.T + n * np.eye(n)
+    B = rng.normal(size=(4, n))
+    B = np.vstack([B, B[0] + B[2]])
+    W, S = oracle.a_orthogonalize(B, A)
+    assert W.shape == (4, n) and np.all(S == 1.0)
+    assert np.abs(W @ A @ W.T - np.eye(4)).max() < 1e-12
+    # the newest row survives unchanged up to scale
+    w_new = W[-1]
+    assert abs(abs(w_new @ B[-1]) / (np.linalg.norm(w_new) * np.linalg.norm(B[-1])) - 1.0) < 1e-12
+    D = np.diag(np.r_[-1.0, np.ones(n - 1)])
+    W2, S2 = oracle.a_orthogonalize(np.eye(n)[:1] * 3.0, D)
+    assert S2.tolist() == [-1.0] and abs(W2[0, 0] - 1.0) < 1e-15

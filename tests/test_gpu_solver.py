@@ -73,3 +73,28 @@ def test_ssr_homo2d_matches_reference_fos(gpu, precision):
           "wall", res.wall_time, flush=True)
     assert abs(res.fos - ref[-1]) < 1e-3
     assert abs(res.lambda_hist[0] - ref[0]) < 1e-12 and abs(res.lambda_hist[1] - ref[1]) < 1e-12
+
+
+def test_a_orthogonalize_matches_oracle_and_drops_dependent_columns(gpu, elastic3d):
+    """Device A-orthonormalisation (no per-column host reads) == the sequential drop rule of
+    linear/orthogonalize.py; an exactly dependent column is dropped; W^T A W = I on the kept set."""
+    import oracle
+    m, op, K, f = elastic3d
+    Ks = K.to_scipy()
+    rng = np.random.default_rng(7)
+    n = K.n_rows
+    free = m.q_mask.T.reshape(-1).astype(np.float64)
+    B = rng.normal(size=(5, n)) * free
+    B = np.vstack([B, B[1] + 2.0 * B[3]])       # newest column depends on two older ones
+    solver = DeflatedFGMRES(PMultigrid(m.elem, m.q_mask, use32=False))
+    solver.setup_preconditioner(K)
+    for row in B:
+        solver.expand_deflation_basis(torch.from_numpy(row).to(gpu))
+    solver.A_orthogonalize(K)
+    W = solver._W.cpu().numpy()
+    Wo, So = oracle.a_orthogonalize(B, Ks)
+    assert W.shape == Wo.shape == (5, n)
+    assert np.abs(W - Wo).max() <= 1e-8 * np.abs(Wo).max()
+    G = W @ (Ks @ W.T)
+    assert np.abs(G - np.eye(5)).max() < 1e-8
+    np.testing.assert_allclose(solver._AW.cpu().numpy(), (Ks @ W.T).T, rtol=0, atol=1e-9 * np.abs(G).max() * 1e3)
