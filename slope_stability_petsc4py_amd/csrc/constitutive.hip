@@ -73,8 +73,9 @@ struct DSWriter {
 // Return mapping at one point. Returns the branch id: 0 elastic, 1 smooth, 2 left edge,
 // 3 right edge, 4 apex.  Follows cython/constitutive_3D_kernel.h:68-351 formula by formula
 // (same thresholds: Q < 1e-30 -> theta = 0, clamped acos argument, exact-tie edge choice).
+template <class Writer>
 __device__ int mc3d_point(const double E[6], double cb, double sp, double G, double lam, double S[6],
-                          const DSWriter &w) {
+                          const Writer &w) {
     const double sq = sp;  // associated flow (sin psi = sin phi)
     const double et[6] = {E[0], E[1], E[2], 0.5 * E[3], 0.5 * E[4], 0.5 * E[5]};
     double esq[6];
@@ -269,22 +270,55 @@ struct MatSource {
     }
 };
 
-__global__ void __launch_bounds__(256) k_constitutive_3d(const double *__restrict__ E, MatSource mat, int64_t n,
-                                                         double *__restrict__ S, DSWriter w,
-                                                         int8_t *__restrict__ branch) {
-    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    double e[6], s[6];
+// Tangent rows of a workgroup's points staged in LDS (row stride 37 doubles), written out to the
+// point-major DS array afterwards by consecutive lanes at consecutive addresses.
+struct LdsDSWriter {
+    double *row;
+    bool enabled;
+    __device__ __forceinline__ void put(int k, double v) const { row[k] = v; }
+    __device__ __forceinline__ bool on() const { return enabled; }
+};
+
+constexpr int kCBlock = 128;   // points per workgroup
+constexpr int kDStride = 37;   // LDS row stride of a point's tangent (36 + 1 against bank conflicts)
+
+// One lane per point.  The block's strains are read through LDS (one coalesced pass), the return
+// mapping writes the point's stress and tangent into LDS rows, and the block's S, DS (and the FP32
+// copy of DS) leave in coalesced passes: the block owns the contiguous ranges
+// S[p0*6 .. (p0+np)*6) and DS[p0*36 .. (p0+np)*36).
+__global__ void __launch_bounds__(kCBlock) k_constitutive_3d(const double *__restrict__ E, MatSource mat, int64_t n,
+                                                             double *__restrict__ S, DSWriter w,
+                                                             int8_t *__restrict__ branch) {
+    __shared__ double sds[kCBlock * kDStride];
+    __shared__ double sse[kCBlock * 7];
+    const int t = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * kCBlock;
+    const int np = (int)(n - p0 < kCBlock ? n - p0 : kCBlock);
+    for (int i = t; i < 6 * np; i += kCBlock) sse[(i / 6) * 7 + i % 6] = E[p0 * 6 + i];
+    __syncthreads();
+    const bool want = w.on();
+    if (t < np) {
+        const int64_t p = p0 + t;
+        double e[6], s[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) e[c] = E[p * 6 + c];
-    double cb, sp, G, lam;
-    mat.get(p, cb, sp, G, lam);
-    DSWriter wp = w;
-    wp.p = p;
-    const int b = mc3d_point(e, cb, sp, G, lam, s, wp);
+        for (int c = 0; c < 6; ++c) e[c] = sse[t * 7 + c];
+        double cb, sp, G, lam;
+        mat.get(p, cb, sp, G, lam);
+        LdsDSWriter lw{sds + t * kDStride, want};
+        const int b = mc3d_point(e, cb, sp, G, lam, s, lw);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) S[p * 6 + c] = s[c];
-    if (branch) branch[p] = (int8_t)b;
+        for (int c = 0; c < 6; ++c) sse[t * 7 + c] = s[c];   // own row only: no hazard with other lanes
+        if (branch) branch[p] = (int8_t)b;
+    }
+    __syncthreads();
+    for (int i = t; i < 6 * np; i += kCBlock) S[p0 * 6 + i] = sse[(i / 6) * 7 + i % 6];
+    if (want) {
+        for (int i = t; i < 36 * np; i += kCBlock) {
+            const double v = sds[(i / 36) * kDStride + i % 36];
+            if (w.ds) w.ds[p0 * 36 + i] = v;
+            if (w.ds32) w.ds32[p0 * 36 + i] = (float)v;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -327,11 +361,22 @@ __device__ int mc2d_point(const double E[3], double cb, double sp, double G, dou
     // sort (problem.py:224-241): test2 -> (a1, a3, a2); test3 -> (a3, a1, a2)
     const bool t2 = (a1 >= a3) && (a3 > a2);
     const bool t3 = a3 > a1;
-    double e1 = a1, e2 = a2, e3 = a3;
-    const double *P1 = P0a, *P2 = P0b, *P3 = P0c;
-    int D1 = 0, D2 = 1, D3 = 2;  // 0 -> +D0a, 1 -> -D0a, 2 -> zero
-    if (t2) { e2 = a3; e3 = a2; P2 = P0c; P3 = P0b; D2 = 2; D3 = 1; }
-    if (t3) { e1 = a3; e2 = a1; e3 = a2; P1 = P0c; P2 = P0a; P3 = P0b; D1 = 2; D2 = 0; D3 = 1; }
+    // t2 and t3 exclude each other (a1 >= a3 vs a3 > a1); the ordered projections are selected by
+    // value (no per-lane choice between arrays, which would put them in scratch memory)
+    const double e1 = t3 ? a3 : a1;
+    const double e2 = t3 ? a1 : (t2 ? a3 : a2);
+    const double e3 = (t2 || t3) ? a2 : a3;
+    double P1[4], P2[4], P3[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        P1[i] = t3 ? P0c[i] : P0a[i];
+        P2[i] = t3 ? P0a[i] : (t2 ? P0c[i] : P0b[i]);
+        P3[i] = (t2 || t3) ? P0b[i] : P0c[i];
+    }
+    // 0 -> +D0a, 1 -> -D0a, 2 -> zero
+    const int D1 = t3 ? 2 : 0;
+    const int D2 = t3 ? 0 : (t2 ? 2 : 1);
+    const int D3 = (t2 || t3) ? 1 : 2;
     auto Dm = [&](int which, int k) -> double { return which == 0 ? D0a[k] : (which == 1 ? -D0a[k] : 0.0); };
 
     const double tr = e1 + e2 + e3;
@@ -519,8 +564,8 @@ int ssx_constitutive_3d(const double *E, const double *c_bar, const double *sin_
     if (n == 0) return SSX_OK;
     MatSource m{c_bar, sin_phi, shear, lame, nullptr, nullptr, 1};
     DSWriter w{DS, DS32, n, 0};
-    hipLaunchKernelGGL(k_constitutive_3d, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), E, m, n,
-                       S, w, branch);
+    hipLaunchKernelGGL(k_constitutive_3d, dim3(ssx::grid_for(n, kCBlock)), dim3(kCBlock), 0, ssx::as_stream(stream), E,
+                       m, n, S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d");
     return SSX_OK;
 }
@@ -532,8 +577,8 @@ int ssx_constitutive_3d_mat(const double *E, const int32_t *mat_id, const double
     if (n == 0) return SSX_OK;
     MatSource m{nullptr, nullptr, nullptr, nullptr, mat_id, table, n_q};
     DSWriter w{DS, DS32, n, 0};
-    hipLaunchKernelGGL(k_constitutive_3d, dim3(ssx::grid_for(n, 256)), dim3(256), 0, ssx::as_stream(stream), E, m, n,
-                       S, w, branch);
+    hipLaunchKernelGGL(k_constitutive_3d, dim3(ssx::grid_for(n, kCBlock)), dim3(kCBlock), 0, ssx::as_stream(stream), E,
+                       m, n, S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d_mat");
     return SSX_OK;
 }

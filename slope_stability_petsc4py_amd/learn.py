@@ -120,6 +120,20 @@ class NodeMLP:
                   stream_ptr())
         return float(part.sum().item()), g
 
+    def save(self, path) -> None:
+        st = self.state_dict()
+        np.savez(path, dims=np.asarray(st["dims"], dtype=np.int64),
+                 **{f"W{i}": w for i, w in enumerate(st["W"])}, **{f"b{i}": b for i, b in enumerate(st["b"])})
+
+    @classmethod
+    def load(cls, path, device=None) -> "NodeMLP":
+        d = np.load(path, allow_pickle=False)
+        dims = [int(x) for x in d["dims"]]
+        net = cls(n_in=dims[0], hidden=dims[1], layers=len(dims) - 1, n_out=dims[-1], device=device)
+        net.load_state_dict({"W": [d[f"W{i}"] for i in range(len(dims) - 1)],
+                             "b": [d[f"b{i}"] for i in range(len(dims) - 1)]})
+        return net
+
     def state_dict(self):
         return {"dims": self.dims, "W": [w.cpu().numpy() for w in self.W], "b": [b.cpu().numpy() for b in self.b]}
 

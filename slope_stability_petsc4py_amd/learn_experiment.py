@@ -37,6 +37,22 @@ def slope_case(seed: int, h: float = 2.0, nz: int = 6) -> cases.Case:
     return c
 
 
+def hetero_variant(seed: int, spread: float = 0.2, dphi: float = 3.0) -> cases.Case:
+    """The 3D heterogeneous L1 slope (P2) with its four layer materials perturbed per seed:
+    c0 scaled by U(1 - spread, 1 + spread), phi shifted by U(-dphi, dphi) degrees.  Seed < 0 gives
+    the unperturbed reference materials."""
+    base = cases.hetero3d_L1()
+    if seed < 0:
+        return base
+    rng = np.random.default_rng(seed)
+    mats = []
+    for m in cases.HETERO_L1_MATERIALS:
+        mats.append(MaterialSpec(c0=m.c0 * float(rng.uniform(1 - spread, 1 + spread)),
+                                 phi=m.phi + float(rng.uniform(-dphi, dphi)), psi=m.psi, young=m.young,
+                                 poisson=m.poisson, gamma_sat=m.gamma_sat, gamma_unsat=m.gamma_unsat))
+    return cases.Case(f"hetero3d_L1_seed{seed}", base.mesh, mats, base.params)
+
+
 def run_case(case, predictor=None, precision="fp64", store=False):
     s = cases.make_solver(case, precision=precision)
     pred = predictor(case) if callable(predictor) else None
@@ -44,8 +60,61 @@ def run_case(case, predictor=None, precision="fp64", store=False):
     return s, res
 
 
+def hetero_main(a, log):
+    """Train on perturbed-material SSR histories of the heterogeneous L1 slope, evaluate on the
+    reference materials: plain FP64 (secant), FP32 (secant), FP32 + learned predictor."""
+    t0 = time.perf_counter()
+    samples, train_info = [], []
+    for seed in range(a.train):
+        case = hetero_variant(seed)
+        s, res = run_case(case, precision="fp32", store=True)
+        info = node_info(case.mesh, case.materials)
+        samples += samples_from_history(info, res.step_U, res.omega_hist, res.lambda_hist)
+        train_info.append({"seed": seed, "fos": res.fos, "newton": res.newton_iterations,
+                           "steps": len(res.lambda_hist), "wall": res.wall_time})
+        log("train run", train_info[-1])
+        del s, res
+        torch.cuda.empty_cache()
+    t_data = time.perf_counter() - t0
+    net = NodeMLP(seed=0)
+    t1 = time.perf_counter()
+    losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    if a.save:
+        net.save(a.save)
+    ref = hetero_variant(-1)
+    # one short warm-up run so every timed run below starts from loaded kernels and libraries
+    warm = cases.make_solver(ref, precision="fp32")
+    ref.params.step_max = 4
+    cases.run_ssr(warm)
+    ref.params.step_max = cases.SSRParams().step_max
+    del warm
+    evals = {}
+    for name, prec, use_net in (("plain_fp64", "fp64", False), ("fp32", "fp32", False),
+                                ("learned_fp32", "fp32", True)):
+        case = hetero_variant(-1)
+        info = node_info(case.mesh, case.materials)
+        s = cases.make_solver(case, precision=prec)
+        pred = LearnedPredictor(net, info) if use_net else None
+        res = cases.run_ssr(s, predictor=pred)
+        used = sum(1 for r in res.step_info if r["predictor"] == "learned")
+        evals[name] = {"fos": res.fos, "wall": res.wall_time, "newton": res.newton_iterations,
+                       "linear": res.linear_iterations, "attempts": res.attempts, "learned_steps_used": used,
+                       "steps": len(res.lambda_hist)}
+        log("eval", name, evals[name])
+        del s, res
+    return {"case": "hetero3d_L1", "train_seeds": a.train, "samples": len(samples), "epochs": a.epochs,
+            "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
+            "data_gen_s": t_data, "train_s": t_train, "train_runs": train_info, "eval": evals,
+            "speedup_learned_fp32_vs_plain_fp64": evals["plain_fp64"]["wall"] / evals["learned_fp32"]["wall"],
+            "newton_ratio_learned_vs_fp32": evals["learned_fp32"]["newton"] / max(1, evals["fp32"]["newton"])}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
+    ap.add_argument("--case", default="homo", choices=["homo", "hetero3d_L1"])
+    ap.add_argument("--save", default=None, help="write the trained weights to this .npz")
     ap.add_argument("--train", type=int, default=12)
     ap.add_argument("--test", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=150)
@@ -53,6 +122,9 @@ def main(argv=None):
     ap.add_argument("--nz", type=int, default=6)
     a = ap.parse_args(argv)
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
+    if a.case == "hetero3d_L1":
+        print(json.dumps(hetero_main(a, log)), flush=True)
+        return
     t0 = time.perf_counter()
     samples = []
     train_info = []

@@ -23,9 +23,12 @@ def main(argv=None):
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--pc", default="pmg", choices=["pmg", "cheb"])
     ap.add_argument("--analysis", default="ssr", choices=["ssr", "ll"])
+    ap.add_argument("--elem-type", default="P2", choices=["P1", "P2", "P4"],
+                    help="element order of the hetero3d_L1 case (P4: the reference's 616,322-dof P4 problem)")
     ap.add_argument("--h", type=float, default=None)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--step-max", type=int, default=None)
+    ap.add_argument("--omega-max", type=float, default=None, help="override omega_max_stop")
     ap.add_argument("--coarse-refresh", type=int, default=0,
                     help="rebuild the multigrid coarse operator every N preconditioner setups (0: when a solve "
                          "needs more than 15 iterations)")
@@ -37,11 +40,13 @@ def main(argv=None):
     if a.case == "homo2d":
         case = cases.homo2d(h=a.h or 1.0)
     elif a.case == "hetero3d_L1":
-        case = cases.hetero3d_L1()
+        case = cases.hetero3d_L1(a.elem_type)
     else:
         case = cases.slope3d(h=a.h or 2.0, nz=a.nz or 10, hetero=a.case.endswith("hetero"))
     if a.step_max:
         case.params.step_max = a.step_max
+    if a.omega_max:
+        case.params.omega_max_stop = a.omega_max
     s = cases.make_solver(case, precision=a.precision, pc=a.pc, coarse_refresh=a.coarse_refresh,
                           coarse_precision=a.coarse_precision)
     torch.cuda.synchronize()
