@@ -55,47 +55,73 @@ __device__ __forceinline__ void brow2(int ci, double x, double y, double b[3]) {
 }
 
 // ------------------------------------------------------------------------------------------
-// Strain gather: one lane per quadrature point.
+// Strain gather: one lane per quadrature point.  A workgroup owns SB consecutive points (128, 64 or
+// 32 as the element grows, so the LDS per workgroup stays near 20-30 KB): their gradient rows (three contiguous slabs of SB*NP values) are
+// staged in LDS with coalesced
+// loads (row stride NP+1 against bank conflicts), each lane then gathers its element's
+// displacements (u is small and stays in L2) and the block's strains leave through LDS in one
+// coalesced pass over the contiguous range E[g0*NS .. (g0+np)*NS).
 // ------------------------------------------------------------------------------------------
-template <int DIM, int NP>
-__global__ void __launch_bounds__(256) k_strain(const double *__restrict__ dphi, const int32_t *__restrict__ edofs,
-                                                const double *__restrict__ u, int64_t n_int, int n_q,
-                                                double *__restrict__ E) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= n_int) return;
-    const int64_t e = g / n_q;
-    const int32_t *dof = edofs + e * (DIM * NP);
-    const double *gx = dphi + g * NP;
-    const double *gy = dphi + (n_int + g) * NP;
-    if constexpr (DIM == 3) {
-        const double *gz = dphi + (2 * n_int + g) * NP;
-        double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
+template <int NP>
+struct StrainBlock {
+    static constexpr int value = NP > 15 ? 32 : (NP > 6 ? 64 : 128);
+};
+
+template <int DIM, int NP, int kSBlock = StrainBlock<NP>::value>
+__global__ void __launch_bounds__(kSBlock) k_strain(const double *__restrict__ dphi, const int32_t *__restrict__ edofs,
+                                                    const double *__restrict__ u, int64_t n_int, int n_q,
+                                                    double *__restrict__ E) {
+    constexpr int NS = BCol<DIM>::NS;
+    constexpr int RS = NP + 1;
+    __shared__ double sg[DIM][kSBlock * RS];
+    __shared__ double se[kSBlock * (NS + 1)];
+    const int t = threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * kSBlock;
+    const int np = (int)(n_int - g0 < kSBlock ? n_int - g0 : kSBlock);
 #pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const double ux = u[dof[3 * j]], uy = u[dof[3 * j + 1]], uz = u[dof[3 * j + 2]];
-            const double x = gx[j], y = gy[j], z = gz[j];
-            a0 += x * ux;
-            a1 += y * uy;
-            a2 += z * uz;
-            a3 += y * ux + x * uy;
-            a4 += z * uy + y * uz;
-            a5 += z * ux + x * uz;
-        }
-        double *o = E + g * 6;
-        o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4; o[5] = a5;
-    } else {
-        double a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll
-        for (int j = 0; j < NP; ++j) {
-            const double ux = u[dof[2 * j]], uy = u[dof[2 * j + 1]];
-            const double x = gx[j], y = gy[j];
-            a0 += x * ux;
-            a1 += y * uy;
-            a2 += y * ux + x * uy;
-        }
-        double *o = E + g * 3;
-        o[0] = a0; o[1] = a1; o[2] = a2;
+    for (int c = 0; c < DIM; ++c) {
+        const double *src = dphi + ((int64_t)c * n_int + g0) * NP;
+        for (int i = t; i < np * NP; i += kSBlock) sg[c][(i / NP) * RS + i % NP] = src[i];
     }
+    __syncthreads();
+    if (t < np) {
+        const int64_t g = g0 + t;
+        const int64_t e = g / n_q;
+        const int32_t *dof = edofs + e * (DIM * NP);
+        const double *gx = sg[0] + t * RS;
+        const double *gy = sg[1] + t * RS;
+        if constexpr (DIM == 3) {
+            const double *gz = sg[2] + t * RS;
+            double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const double ux = u[dof[3 * j]], uy = u[dof[3 * j + 1]], uz = u[dof[3 * j + 2]];
+                const double x = gx[j], y = gy[j], z = gz[j];
+                a0 += x * ux;
+                a1 += y * uy;
+                a2 += z * uz;
+                a3 += y * ux + x * uy;
+                a4 += z * uy + y * uz;
+                a5 += z * ux + x * uz;
+            }
+            double *o = se + t * (NS + 1);
+            o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4; o[5] = a5;
+        } else {
+            double a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+            for (int j = 0; j < NP; ++j) {
+                const double ux = u[dof[2 * j]], uy = u[dof[2 * j + 1]];
+                const double x = gx[j], y = gy[j];
+                a0 += x * ux;
+                a1 += y * uy;
+                a2 += y * ux + x * uy;
+            }
+            double *o = se + t * (NS + 1);
+            o[0] = a0; o[1] = a1; o[2] = a2;
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < np * NS; i += kSBlock) E[g0 * NS + i] = se[(i / NS) * (NS + 1) + i % NS];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -301,7 +327,8 @@ int launch_tangent(const double *dphi, const double *DS, const double *w, const 
 template <int DIM, int NP>
 int launch_strain(const double *dphi, const int32_t *edofs, const double *u, int64_t n_int, int n_q, double *E,
                   hipStream_t st) {
-    hipLaunchKernelGGL((k_strain<DIM, NP>), dim3(ssx::grid_for(n_int, 256)), dim3(256), 0, st, dphi, edofs, u, n_int,
+    constexpr int SB = StrainBlock<NP>::value;
+    hipLaunchKernelGGL((k_strain<DIM, NP>), dim3(ssx::grid_for(n_int, SB)), dim3(SB), 0, st, dphi, edofs, u, n_int,
                        n_q, E);
     SSX_CHECK_LAUNCH("ssx_strain");
     return SSX_OK;
