@@ -239,6 +239,46 @@ def cpu_baseline(op, E_host, ab, budget_s: float):
     }
 
 
+def run_ssr_leg(log_progress=True) -> dict:
+    """End-to-end leg of the metric: the reference's 3D heterogeneous P2 SSR (SSR_hetero_ada_L1,
+    four layers, 82,815 dofs) to the factor of safety, plain FP64 GPU form (FP64 preconditioner,
+    secant predictor) against the FP32-preconditioner form with the learned predictor (weights
+    trained on the GPU by the project's kernels, `slope_stability_petsc4py_amd/data/`).  Wall time =
+    continuation from the first Newton solve to the last accepted step, inputs on the device."""
+    import torch
+
+    from slope_stability_petsc4py_amd import cases
+    from slope_stability_petsc4py_amd.learn import LearnedPredictor, NodeMLP, node_info
+
+    fos_ref = float(np.load(ROOT / "tests" / "golden" / "ssr_histories.npz")["hetero3d_lambda_hist"][-1])
+    weights = ROOT / "slope_stability_petsc4py_amd" / "data" / "predictor_hetero3d_L1.npz"
+    case = cases.hetero3d_L1()
+    # warm-up: a short run loads every kernel and library once
+    case.params.step_max = 4
+    cases.run_ssr(cases.make_solver(case, precision="fp32"))
+    case.params.step_max = cases.SSRParams().step_max
+    out = {"workload": "3D heterogeneous P2 SSR (SSR_hetero_ada_L1, 18,419 tets, 82,815 dofs)", "fos_reference": fos_ref}
+    net = NodeMLP.load(weights) if weights.exists() else None
+    forms = [("plain_fp64", "fp64", False), ("fp32", "fp32", False)]
+    if net is not None:
+        forms.append(("learned_fp32", "fp32", True))
+    for name, prec, learned in forms:
+        s = cases.make_solver(case, precision=prec)
+        pred = LearnedPredictor(net, node_info(case.mesh, case.materials)) if learned else None
+        torch.cuda.synchronize()
+        res = cases.run_ssr(s, predictor=pred)
+        out[name] = {"wall_s": round(res.wall_time, 3), "fos": res.fos, "fos_abs_err": abs(res.fos - fos_ref),
+                     "newton": res.newton_iterations, "linear": res.linear_iterations,
+                     "learned_steps": sum(1 for r in res.step_info if r["predictor"] == "learned")}
+        if log_progress:
+            _log(f"ssr {name}: {out[name]}")
+        del s, res
+    best = "learned_fp32" if "learned_fp32" in out else "fp32"
+    out["speedup_vs_plain_fp64"] = round(out["plain_fp64"]["wall_s"] / out[best]["wall_s"], 3)
+    out["fast_form"] = best
+    return out
+
+
 def run_p4(args):
     """P4 assembly on the matrix cores: 3D heterogeneous slope (SSR_hetero_ada_L2, 33,935 tets elevated
     to P4, 24-point rule, four layer materials), return mapping + element matrices (FP64 MFMA) +
@@ -251,10 +291,7 @@ def run_p4(args):
     from slope_stability_petsc4py_amd.device import ptr, stream_ptr
 
     t0 = time.time()
-    d = np.load(ROOT / "tests" / "golden" / "ssr_hetero_L2_tet4.npz", allow_pickle=False)
-    m = M.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
-                               d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,
-                               {"x": [1, 2], "y": [5], "z": [3, 4]}, "P4")
+    m = cases.gmsh_fixture_mesh("ssr_hetero_L2_tet4.npz", "P4")
     op = ConstitutiveOperator.from_mesh(m, workloads.HETERO_LAYERS, Davis_type="B")
     pat = op.pattern
     _log(f"P4 mesh+pattern {time.time() - t0:.1f}s: {m.n_elem} tets, {op.n_int} points, {pat.n_rows} dofs, "
@@ -344,6 +381,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--no-ssr", action="store_true", help="skip the end-to-end SSR leg")
     ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows", "elem"],
                     help="P2 tangent path: element matrices + node gather (default), fused row kernel, or "
                          "element blocks + row gather")
@@ -365,6 +403,10 @@ def main():
     if not args.no_cpu:
         _log("cpu baseline (reference C on host cores)...")
         cpu = cpu_baseline(op, res["E"], res["ab"], args.cpu_budget)
+    ssr = None
+    if not args.no_ssr:
+        _log("end-to-end SSR leg (3D heterogeneous P2)...")
+        ssr = run_ssr_leg()
     cpu_model = ""
     try:
         cpu_model = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -396,6 +438,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": res["traffic"],
         },
         "cpu_baseline": cpu,
+        "ssr": ssr,
         "kernels": kern,
         "host": {"cpu_model": cpu_model, "nproc": os.cpu_count(), "python": platform.python_version()},
     }

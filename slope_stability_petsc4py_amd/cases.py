@@ -69,13 +69,56 @@ def homo2d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.0)
     return Case("homo2d", m, [HOMO], p)
 
 
+def gmsh_fixture_mesh(fixture: str, elem_type: str = "P2") -> M.SlopeMesh:
+    """A committed Gmsh-family mesh (tests/golden/*_tet4.npz): logical material and boundary ids
+    from the file's physical names, Dirichlet labels x: [1, 2], y: [5], z: [3, 4] (the families'
+    definition.py)."""
+    d = np.load(DATA / fixture, allow_pickle=False)
+    return M.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_label"].astype(np.int64),
+                                  d["tri3"].astype(np.int64), d["tri_label"].astype(np.int64),
+                                  {"x": [1, 2], "y": [5], "z": [3, 4]}, elem_type)
+
+
 def hetero3d_L1(elem_type: str = "P2") -> Case:
     """elem_type "P4" gives the reference's P4 L1 problem (616,322 free dofs)."""
-    d = np.load(DATA / "ssr_hetero_L1_tet4.npz", allow_pickle=False)
-    m = M.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
-                               d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,
-                               {"x": [1, 2], "y": [5], "z": [3, 4]}, elem_type)
-    return Case("hetero3d_L1", m, HETERO_L1_MATERIALS, SSRParams())
+    return Case("hetero3d_L1", gmsh_fixture_mesh("ssr_hetero_L1_tet4.npz", elem_type), HETERO_L1_MATERIALS,
+                SSRParams())
+
+
+HOMO_LL_MATERIAL = MaterialSpec(6.0, 45.0, 0.0, 40000.0, 0.30, 20.0, 20.0)
+
+
+def _ll_params(d_omega_ini_scale: float, omega_max: float, it_newt_max: int) -> SSRParams:
+    p = SSRParams(step_max=100, omega_max_stop=omega_max, it_newt_max=it_newt_max)
+    p.d_omega_ini_scale = d_omega_ini_scale
+    p.d_t_min = 1e-3
+    return p
+
+
+def _gmsh_case(name: str, fixture: str, materials, elem_type: str, params) -> Case:
+    return Case(name, gmsh_fixture_mesh(fixture, elem_type), materials, params)
+
+
+def homo2d_ll(h: float = 0.5) -> Case:
+    """The reference's 2D homogeneous LL (benchmarks/slope_stability_2D_homo_LL/case.toml: h = 0.5,
+    x1 = x3 = 15, d_omega_ini_scale 0.2, omega_max 2000, it_newt_max 100)."""
+    m = M.slope_2d_p2(h=h, x1=15.0, x2=10.0, x3=15.0, y1=10.0, y2=10.0)
+    return Case("homo2d_ll", m, [HOMO], _ll_params(0.2, 2000.0, 100))
+
+
+def homo3d_ll_L1(elem_type: str = "P2") -> Case:
+    """The reference's 3D homogeneous LL on LL_homo_ada_L1 (benchmarks/slope_stability_3D_homo_LL:
+    d_omega_ini_scale 0.2, omega_max 4900, it_newt_max 200)."""
+    return _gmsh_case("homo3d_ll_L1", "ll_homo_L1_tet4.npz", [HOMO_LL_MATERIAL], elem_type,
+                      _ll_params(0.2, 4900.0, 200))
+
+
+def hetero3d_ll_L1(elem_type: str = "P2") -> Case:
+    """The reference's 3D heterogeneous LL on LL_hetero_ada_L1 (benchmarks/slope_stability_3D_hetero_LL:
+    the four layer materials of meshes/3d_hetero_ll/definition.py, d_omega_ini_scale 0.5,
+    omega_max 8e7, it_newt_max 200)."""
+    return _gmsh_case("hetero3d_ll_L1", "ll_hetero_L1_tet4.npz", HETERO_L1_MATERIALS, elem_type,
+                      _ll_params(0.5, 8.0e7, 200))
 
 
 def slope3d(h: float = 2.0, nz: int = 10, hetero: bool = False) -> Case:
@@ -125,8 +168,12 @@ def run_ssr(s: Solver, *, predictor=None, store_step_u: bool = False, progress=N
                                      predictor=predictor, store_step_u=store_step_u, progress=progress)
 
 
-def run_ll(s: Solver, *, d_omega_ini_scale: float = 0.2, d_t_min: float = 1e-3, progress=None):
+def run_ll(s: Solver, *, d_omega_ini_scale: float | None = None, d_t_min: float | None = None, progress=None):
     p = s.case.params
+    if d_omega_ini_scale is None:
+        d_omega_ini_scale = getattr(p, "d_omega_ini_scale", 0.2)
+    if d_t_min is None:
+        d_t_min = getattr(p, "d_t_min", 1e-3)
     return ll_indirect_continuation(d_omega_ini_scale, d_t_min, p.step_max, p.omega_max_stop, p.it_newt_max,
                                     p.it_damp_max, p.tol, p.r_min, s.f, s.op, s.linear, free_mask=s.free_mask,
                                     dim=s.case.mesh.dim, progress=progress)

@@ -240,46 +240,60 @@ __device__ __forceinline__ void ke_pipe_body(const double *__restrict__ dphi, co
             const double *gq = L + q * 3 * 2 * HP;
             const double xi = gq[(0 * 2 + hi) * HP + ki], yi = gq[(1 * 2 + hi) * HP + ki], zi = gq[(2 * 2 + hi) * HP + ki];
             const double wg = L[P::G + P::D + q];
+            // B row a has coefficients (c0, c1, c2) on strain rows (s0, s1, s2); they are scaled by
+            // w (1 - r) once, so t = w (1 - r) D b_a costs 18 FMAs
+            const double c0 = ci == 0 ? xi : (ci == 1 ? yi : zi);
+            const double c1 = ci == 1 ? xi : yi;
+            const double c2 = ci == 2 ? xi : zi;
+            const double wo = wg * om;
+            const double wc0 = wo * c0, wc1 = wo * c1, wc2 = wo * c2;
             double t[6];
             if constexpr (DSMEM) {
                 const double *Dg = DS + (e * NQ + q) * 36;   // column-major D[s + 6 m]
-                double b[6];
-                brow(ci, xi, yi, zi, b);
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
-                    double v = b[0] * Dg[6 * m];
+                    // rows s0, s1, s2 of column m, selected by value from the six entries
+                    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 #pragma unroll
-                    for (int ss = 1; ss < 6; ++ss) v += b[ss] * Dg[ss + 6 * m];
-                    t[m] = v;
+                    for (int ss = 0; ss < 6; ++ss) {
+                        const double v = Dg[ss + 6 * m];
+                        d0 = ss == s0 ? v : d0;
+                        d1 = ss == s1 ? v : d1;
+                        d2 = ss == s2 ? v : d2;
+                    }
+                    t[m] = fma(wc2, d2, fma(wc1, d1, wc0 * d0));
                 }
             } else {
-                // B row a has coefficients (c0, c1, c2) on strain rows (s0, s1, s2)
-                const double c0 = ci == 0 ? xi : (ci == 1 ? yi : zi);
-                const double c1 = ci == 1 ? xi : yi;
-                const double c2 = ci == 2 ? xi : zi;
                 const double *D0 = L + P::G + q * 36 + s0 * 6;
                 const double *D1 = L + P::G + q * 36 + s1 * 6;
                 const double *D2 = L + P::G + q * 36 + s2 * 6;
 #pragma unroll
-                for (int m = 0; m < 6; ++m) t[m] = c0 * D0[m] + c1 * D1[m] + c2 * D2[m];
+                for (int m = 0; m < 6; ++m) t[m] = fma(wc2, D2[m], fma(wc1, D1[m], wc0 * D0[m]));
             }
             if constexpr (FOLD) {
+                // + w r D_el b_a with D_el b = lame (b0 + b1 + b2) vol + 2 G ident b; here
+                // b0 + b1 + b2 = c0 and b is (c0, c1, c2) on rows (s0, s1, s2)
                 const double Gq = L[P::G + P::D + NQ + q], Lq = L[P::G + P::D + 2 * NQ + q];
-                double b[6];
-                brow(ci, xi, yi, zi, b);
-                const double tr = Lq * (b[0] + b[1] + b[2]);
-#pragma unroll
-                for (int m = 0; m < 6; ++m) t[m] = om * t[m] + r * (m < 3 ? tr + 2.0 * Gq * b[m] : Gq * b[m]);
+                const double rw = r * wg;
+                const double tr = Lq * c0, g2 = 2.0 * Gq;
+                const double b3 = ci != 2 ? c1 : 0.0;
+                const double b4 = ci == 2 ? c1 : (ci == 1 ? c2 : 0.0);
+                const double b5 = ci != 1 ? c2 : 0.0;
+                t[0] = fma(rw, fma(g2, ci == 0 ? c0 : 0.0, tr), t[0]);
+                t[1] = fma(rw, fma(g2, ci == 1 ? c0 : 0.0, tr), t[1]);
+                t[2] = fma(rw, fma(g2, ci == 2 ? c0 : 0.0, tr), t[2]);
+                const double rg = rw * Gq;
+                t[3] = fma(rg, b3, t[3]);
+                t[4] = fma(rg, b4, t[4]);
+                t[5] = fma(rg, b5, t[5]);
             }
-#pragma unroll
-            for (int m = 0; m < 6; ++m) t[m] *= wg;
             const double *gx = gq + (0 * 2 + h) * HP, *gy = gq + (1 * 2 + h) * HP, *gz = gq + (2 * 2 + h) * HP;
 #pragma unroll
             for (int k = 0; k < NH; ++k) {
                 const double xj = gx[k], yj = gy[k], zj = gz[k];
-                acc[3 * k + 0] += t[0] * xj + t[3] * yj + t[5] * zj;
-                acc[3 * k + 1] += t[3] * xj + t[1] * yj + t[4] * zj;
-                acc[3 * k + 2] += t[5] * xj + t[4] * yj + t[2] * zj;
+                acc[3 * k + 0] = fma(t[5], zj, fma(t[3], yj, fma(t[0], xj, acc[3 * k + 0])));
+                acc[3 * k + 1] = fma(t[4], zj, fma(t[1], yj, fma(t[3], xj, acc[3 * k + 1])));
+                acc[3 * k + 2] = fma(t[2], zj, fma(t[4], yj, fma(t[5], xj, acc[3 * k + 2])));
             }
         }
         if (valid) {

@@ -4,9 +4,11 @@ HIP kernels (bf16 matrix-core GEMMs with FP32 accumulation, tanh, Adam).
 It fills the predictor slot of the SSR continuation (continuation/indirect.py:2703-2830).  The
 network predicts a correction to the secant increment, per node:
 
-    dU_pred(node) = a * dU_{k-1}(node) + s * MLP(features(node))
+    dU_pred(node) = sec(node) + loc(node) * smooth(MLP(features(node)))
 
-with a = (omega_target - omega) / (omega - omega_old) the secant factor and s = max |dU_{k-1}|.
+with sec = a * dU_{k-1} the secant increment, a = (omega_target - omega) / (omega - omega_old),
+loc = |sec| + 0.02 * |a| * s the local scale (s = max |dU_{k-1}|) and smooth a few passes of
+element/nodal averaging over the mesh.
 Features (16 per node): normalised coordinates (3), the node's material strength (c0 scaled,
 tan phi) (2), lambda and its last increment (2), dU_{k-1}/s, dU_{k-2}/s and the secant increment
 a*dU_{k-1}/s (3 each, zero-padded in 2D).  The last layer starts at zero, so an untrained network
@@ -156,6 +158,8 @@ class NodeInfo:
     mat: torch.Tensor       # (n_nodes, 2)
     free: torch.Tensor      # (n_nodes, 3) float mask (zero-padded in 2D)
     dim: int
+    elem: torch.Tensor | None = None       # (n_elem, n_p) int64 node table (smoothing)
+    inv_count: torch.Tensor | None = None  # (n_nodes, 1) 1 / number of elements at each node
 
 
 def node_info(mesh, materials, device=None) -> NodeInfo:
@@ -175,7 +179,41 @@ def node_info(mesh, materials, device=None) -> NodeInfo:
     free = np.zeros((mesh.n_nodes, 3))
     free[:, : mesh.dim] = mesh.q_mask.T
     t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(dev)  # noqa: E731
-    return NodeInfo(t(c), t(nm), t(free), mesh.dim)
+    elem = np.ascontiguousarray(np.asarray(mesh.elem, dtype=np.int64).T)
+    cnt = np.bincount(elem.reshape(-1), minlength=mesh.n_nodes).astype(np.float32)
+    return NodeInfo(t(c), t(nm), t(free), mesh.dim, torch.from_numpy(elem).to(dev),
+                    t((1.0 / np.maximum(cnt, 1.0))[:, None]))
+
+
+def smooth_nodal(info: NodeInfo, v: torch.Tensor, passes: int) -> torch.Tensor:
+    """Low-pass filter of a nodal field (n_nodes, 3): `passes` rounds of element averaging
+    (mean over an element's nodes) followed by nodal averaging over the elements at each node."""
+    if passes <= 0 or info.elem is None:
+        return v
+    flat = info.elem.reshape(-1)
+    n_p = info.elem.shape[1]
+    for _ in range(passes):
+        ve = v[info.elem].mean(dim=1)                                  # (n_elem, 3)
+        acc = torch.zeros_like(v)
+        acc.index_add_(0, flat, ve.repeat_interleave(n_p, dim=0))
+        v = acc * info.inv_count
+    return v
+
+
+def smooth_nodal_T(info: NodeInfo, g: torch.Tensor, passes: int) -> torch.Tensor:
+    """Transpose of `smooth_nodal` (for gradients through the smoother): one pass of smooth_nodal
+    is S = diag(1/count) E^T E / n_p with E the element-node incidence, so
+    S^T g = E^T E (g / count) / n_p."""
+    if passes <= 0 or info.elem is None:
+        return g
+    flat = info.elem.reshape(-1)
+    n_p = info.elem.shape[1]
+    for _ in range(passes):
+        ge = (g * info.inv_count)[info.elem].sum(dim=1) / n_p      # (n_elem, 3)
+        acc = torch.zeros_like(g)
+        acc.index_add_(0, flat, ge.repeat_interleave(n_p, dim=0))
+        g = acc
+    return g
 
 
 def _pad3(v: torch.Tensor, dim: int) -> torch.Tensor:
@@ -186,12 +224,22 @@ def _pad3(v: torch.Tensor, dim: int) -> torch.Tensor:
     return torch.nn.functional.pad(x, (0, 3 - dim))
 
 
+LOCAL_FLOOR = 0.02   # floor of the local scale, as a fraction of the largest increment
+
+
 def features(info: NodeInfo, dU1: torch.Tensor, dU2: torch.Tensor, a: float, lam: float, dlam: float):
-    """(X (n_nodes, 16), scale s, secant increment (n_nodes, 3))."""
+    """(X (n_nodes, 16), scale s, secant increment (n_nodes, 3), local scale (n_nodes, 3)).
+
+    The network's output is a correction RELATIVE to the local secant increment: the increment is
+    sec + loc * out with loc = |sec| + LOCAL_FLOOR * s, so the error of the prediction shrinks with
+    the local motion (far from the failure zone the guess stays close to the secant instead of
+    carrying errors of the size of the largest increment, which the stiffness would amplify in the
+    residual)."""
     d1 = _pad3(dU1, info.dim)
     d2 = _pad3(dU2, info.dim)
     s = float(d1.abs().max().item()) or 1.0
     sec = a * d1
+    loc = sec.abs() + LOCAL_FLOOR * abs(a) * s + 1e-30
     n = d1.shape[0]
     X = torch.empty((n, N_FEAT), dtype=torch.float32, device=d1.device)
     X[:, 0:3] = info.coord
@@ -201,7 +249,7 @@ def features(info: NodeInfo, dU1: torch.Tensor, dU2: torch.Tensor, a: float, lam
     X[:, 7:10] = d1 / s
     X[:, 10:13] = d2 / s
     X[:, 13:16] = sec / s
-    return X, s, sec
+    return X, s, sec, loc
 
 
 def samples_from_history(info: NodeInfo, step_U, omega_hist, lambda_hist):
@@ -212,19 +260,28 @@ def samples_from_history(info: NodeInfo, step_U, omega_hist, lambda_hist):
         dU2, dU1, dU_next = U1 - U0, U2 - U1, U3 - U2
         den = omega_hist[k] - omega_hist[k - 1]
         a = 0.0 if abs(den) < 1e-30 else (omega_hist[k + 1] - omega_hist[k]) / den
-        X, s, sec = features(info, dU1, dU2, a, float(lambda_hist[k]), float(lambda_hist[k] - lambda_hist[k - 1]))
-        target = (_pad3(dU_next, info.dim) - sec) / s
+        X, s, sec, loc = features(info, dU1, dU2, a, float(lambda_hist[k]), float(lambda_hist[k] - lambda_hist[k - 1]))
+        target = torch.clamp((_pad3(dU_next, info.dim) - sec) / loc, -4.0, 4.0)
         out.append((X, target.contiguous(), info.free))
     return out
 
 
-def train(net: NodeMLP, samples, *, epochs: int = 200, lr: float = 2e-3, log=None) -> list[float]:
+def train(net: NodeMLP, samples, *, epochs: int = 200, lr: float = 2e-3, log=None, info: NodeInfo | None = None,
+          smoothing: int = 0) -> list[float]:
+    """Adam on the masked MSE.  With `info` and `smoothing` > 0 the loss is taken on the smoothed
+    prediction (the field the predictor actually adds), the gradient flowing back through the
+    transpose of the smoother."""
     losses = []
     for ep in range(epochs):
         tot = 0.0
         for X, T, mask in samples:
             pred = net.forward(X)
-            loss, g = net.mse(pred, T, mask)
+            if info is not None and smoothing > 0:
+                sp = smooth_nodal(info, pred, smoothing).contiguous()
+                loss, g = net.mse(sp, T, mask)
+                g = smooth_nodal_T(info, g, smoothing).contiguous()
+            else:
+                loss, g = net.mse(pred, T, mask)
             net.backward(g)
             net.adam(lr)
             tot += loss
@@ -235,11 +292,14 @@ def train(net: NodeMLP, samples, *, epochs: int = 200, lr: float = 2e-3, log=Non
 
 
 class LearnedPredictor:
-    """Continuation hook: returns U + predicted increment, or None while fewer than 3 states exist."""
+    """Continuation hook: returns U + predicted increment, or None while fewer than 3 states exist.
+    The increment is the secant increment plus the network's relative correction (smoothed over
+    the mesh by `smoothing` passes of element/nodal averaging) times the local secant scale."""
 
-    def __init__(self, net: NodeMLP, info: NodeInfo):
+    def __init__(self, net: NodeMLP, info: NodeInfo, smoothing: int = 2):
         self.net = net
         self.info = info
+        self.smoothing = int(smoothing)
         self.calls = 0
 
     def __call__(self, ctx):
@@ -249,9 +309,11 @@ class LearnedPredictor:
         (U0, w0, l0), (U1, w1, l1), (U2, w2, l2) = hist[-3], hist[-2], hist[-1]
         den = w2 - w1
         a = 0.0 if abs(den) < 1e-30 else (ctx["omega_target"] - w2) / den
-        X, s, sec = features(self.info, U2 - U1, U1 - U0, a, l2, l2 - l1)
-        corr = self.net.forward(X, keep=False)
-        inc = (sec + s * corr) * self.info.free
+        X, s, sec, loc = features(self.info, U2 - U1, U1 - U0, a, l2, l2 - l1)
+        # per-node predictions carry point-to-point noise; the relative correction is low-pass
+        # filtered over the mesh before it is scaled by the local secant increment
+        corr = smooth_nodal(self.info, self.net.forward(X, keep=False), self.smoothing)
+        inc = (sec + loc * corr) * self.info.free
         self.calls += 1
         dim = self.info.dim
         return U2 + inc[:, :dim].reshape(-1).to(torch.float64)

@@ -65,7 +65,7 @@ def hetero_main(a, log):
     reference materials: plain FP64 (secant), FP32 (secant), FP32 + learned predictor."""
     t0 = time.perf_counter()
     samples, train_info = [], []
-    for seed in range(a.train):
+    for seed in range(0 if a.load else a.train):
         case = hetero_variant(seed)
         s, res = run_case(case, precision="fp32", store=True)
         info = node_info(case.mesh, case.materials)
@@ -76,9 +76,15 @@ def hetero_main(a, log):
         del s, res
         torch.cuda.empty_cache()
     t_data = time.perf_counter() - t0
-    net = NodeMLP(seed=0)
     t1 = time.perf_counter()
-    losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l))
+    if a.load:
+        net = NodeMLP.load(a.load)
+        losses = []
+    else:
+        net = NodeMLP(seed=0)
+        info0 = node_info(hetero_variant(-1).mesh, cases.HETERO_L1_MATERIALS)
+        losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l),
+                       info=info0 if a.train_smoothing else None, smoothing=a.train_smoothing)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     if a.save:
@@ -91,30 +97,40 @@ def hetero_main(a, log):
     ref.params.step_max = cases.SSRParams().step_max
     del warm
     evals = {}
-    for name, prec, use_net in (("plain_fp64", "fp64", False), ("fp32", "fp32", False),
-                                ("learned_fp32", "fp32", True)):
+    runs = [("plain_fp64", "fp64", None), ("fp32", "fp32", None)]
+    runs += [(f"learned_fp32_s{k}", "fp32", k) for k in a.smoothing]
+    for name, prec, smooth in runs:
         case = hetero_variant(-1)
         info = node_info(case.mesh, case.materials)
         s = cases.make_solver(case, precision=prec)
-        pred = LearnedPredictor(net, info) if use_net else None
+        pred = LearnedPredictor(net, info, smoothing=smooth) if smooth is not None else None
         res = cases.run_ssr(s, predictor=pred)
         used = sum(1 for r in res.step_info if r["predictor"] == "learned")
+        gates = [(round(r["secant_first_residual"], 3), round(r["learned_first_residual"], 3))
+                 for r in res.step_info if "secant_first_residual" in r]
         evals[name] = {"fos": res.fos, "wall": res.wall_time, "newton": res.newton_iterations,
                        "linear": res.linear_iterations, "attempts": res.attempts, "learned_steps_used": used,
-                       "steps": len(res.lambda_hist)}
+                       "steps": len(res.lambda_hist), "gate_secant_vs_learned": gates}
         log("eval", name, evals[name])
         del s, res
+    best = min((k for k in evals if k.startswith("learned")), key=lambda k: evals[k]["wall"], default=None)
     return {"case": "hetero3d_L1", "train_seeds": a.train, "samples": len(samples), "epochs": a.epochs,
             "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
             "data_gen_s": t_data, "train_s": t_train, "train_runs": train_info, "eval": evals,
-            "speedup_learned_fp32_vs_plain_fp64": evals["plain_fp64"]["wall"] / evals["learned_fp32"]["wall"],
-            "newton_ratio_learned_vs_fp32": evals["learned_fp32"]["newton"] / max(1, evals["fp32"]["newton"])}
+            "best_learned": best,
+            "speedup_learned_fp32_vs_plain_fp64": evals["plain_fp64"]["wall"] / evals[best]["wall"] if best else None,
+            "newton_ratio_learned_vs_fp32": evals[best]["newton"] / max(1, evals["fp32"]["newton"]) if best else None}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", default="homo", choices=["homo", "hetero3d_L1"])
     ap.add_argument("--save", default=None, help="write the trained weights to this .npz")
+    ap.add_argument("--load", default=None, help="skip training: evaluate these weights")
+    ap.add_argument("--train-smoothing", type=int, default=0,
+                    help="train on the prediction smoothed by this many passes (hetero3d_L1)")
+    ap.add_argument("--smoothing", type=int, nargs="+", default=[0, 3, 8],
+                    help="smoothing passes of the learned correction to evaluate")
     ap.add_argument("--train", type=int, default=12)
     ap.add_argument("--test", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=150)
@@ -138,9 +154,15 @@ def main(argv=None):
         del s, res
         torch.cuda.empty_cache()
     t_data = time.perf_counter() - t0
-    net = NodeMLP(seed=0)
     t1 = time.perf_counter()
-    losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l))
+    if a.load:
+        net = NodeMLP.load(a.load)
+        losses = []
+    else:
+        net = NodeMLP(seed=0)
+        info0 = node_info(hetero_variant(-1).mesh, cases.HETERO_L1_MATERIALS)
+        losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l),
+                       info=info0 if a.train_smoothing else None, smoothing=a.train_smoothing)
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     evals = []

@@ -372,6 +372,42 @@ def read_gmsh22(path: str | Path):
             tri3, np.concatenate(tri_tag).astype(np.int64) if tri_tag else np.empty(0, dtype=np.int64))
 
 
+def read_gmsh_physical_names(path: str | Path) -> dict[tuple[int, int], str]:
+    """{(dim, physical tag): name} of a Gmsh 2.2 file's $PhysicalNames section ({} if absent)."""
+    data = Path(path).read_bytes()
+    at = data.find(b"$PhysicalNames")
+    if at < 0:
+        return {}
+    end = data.index(b"$EndPhysicalNames", at)
+    lines = data[at:end].decode(errors="replace").splitlines()[2:]
+    out = {}
+    for ln in lines:
+        parts = ln.split(None, 2)
+        if len(parts) == 3:
+            out[(int(parts[0]), int(parts[1]))] = parts[2].strip().strip('"')
+    return out
+
+
+def logical_ids(tags: np.ndarray, names: dict[tuple[int, int], str], dim: int, prefix: str) -> np.ndarray:
+    """Physical tags -> the logical ids the reference uses (io.py:165-189): a group named
+    "<prefix>_<k>" of this dimension stands for id k; without such names the tags are the ids."""
+    mapping = {}
+    for (d, tag), name in names.items():
+        low = name.strip().lower()
+        if d == dim and low.startswith(prefix + "_") and low[len(prefix) + 1:].isdigit():
+            mapping[tag] = int(low[len(prefix) + 1:])
+    ids = np.asarray(tags, dtype=np.int64).ravel()
+    if not mapping:
+        return ids
+    missing = sorted(int(v) for v in np.unique(ids) if int(v) not in mapping)
+    if missing:
+        raise ValueError(f"physical groups {missing} have no '{prefix}_<k>' name")
+    lut = np.zeros(max(mapping) + 1, dtype=np.int64)
+    for tag, k in mapping.items():
+        lut[tag] = k
+    return lut[ids]
+
+
 _P4_FACE_LOCAL = {  # local P4 node indices of each tetrahedron face (vertices, edge nodes, face nodes)
     (0, 1, 2): [0, 1, 2, 4, 5, 6, 7, 8, 9, 10, 11, 12, 22, 23, 24],
     (0, 1, 3): [0, 1, 3, 4, 5, 6, 13, 14, 15, 19, 20, 21, 25, 26, 27],

@@ -19,7 +19,8 @@ from . import cases
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", default="homo2d", choices=["homo2d", "hetero3d_L1", "slope3d_homo", "slope3d_hetero"])
+    ap.add_argument("--case", default="homo2d", choices=["homo2d", "hetero3d_L1", "slope3d_homo", "slope3d_hetero",
+                                                         "homo2d_ll", "homo3d_ll_L1", "hetero3d_ll_L1"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--pc", default="pmg", choices=["pmg", "cheb"])
     ap.add_argument("--analysis", default="ssr", choices=["ssr", "ll"])
@@ -37,8 +38,16 @@ def main(argv=None):
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
+    if a.case.endswith("_ll") or a.case.endswith("_ll_L1"):
+        a.analysis = "ll"
     if a.case == "homo2d":
         case = cases.homo2d(h=a.h or 1.0)
+    elif a.case == "homo2d_ll":
+        case = cases.homo2d_ll(h=a.h or 0.5)
+    elif a.case == "homo3d_ll_L1":
+        case = cases.homo3d_ll_L1(a.elem_type)
+    elif a.case == "hetero3d_ll_L1":
+        case = cases.hetero3d_ll_L1(a.elem_type)
     elif a.case == "hetero3d_L1":
         case = cases.hetero3d_L1(a.elem_type)
     else:
@@ -71,8 +80,12 @@ def main(argv=None):
     else:
         res = cases.run_ll(s, progress=prog)
         out = {"case": case.name, "precision": a.precision, "limit_load": res.limit_load,
-               "t_hist": res.t_hist.tolist(), "wall_time_s": res.wall_time, "stop_reason": res.stop_reason,
-               "newton_iterations": res.newton_iterations, "linear_iterations": res.linear_iterations}
+               "t_hist": res.t_hist.tolist(), "omega_hist": res.omega_hist.tolist(),
+               "Umax_hist": res.Umax_hist.tolist(), "wall_time_s": res.wall_time, "setup_s": t_setup,
+               "stop_reason": res.stop_reason, "newton_iterations": res.newton_iterations,
+               "linear_iterations": res.linear_iterations, "n_dof": int(s.op.pattern.n_rows),
+               "n_elem": int(case.mesh.n_elem), "linear_solver": s.linear.iteration_collector.summary(),
+               "pc_timings": getattr(s.linear.pc, "timings", None)}
     print(json.dumps(out), flush=True)
 
 

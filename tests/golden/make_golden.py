@@ -8,6 +8,10 @@
   kernels (cython/assemble_tangent_values_3d.c) on them.
 * ssr_histories.npz   -- data fields of the reference's committed SSR runs (lambda / omega / Umax
   histories of benchmarks/run_2D_homo_SSR_capture and benchmarks/run_3D_hetero_SSR_capture).
+* ll_histories.npz    -- the same fields of its committed limit-load runs (2D homo, 3D homo, 3D hetero).
+* *_tet4.npz          -- the reference's Gmsh tet4 meshes (SSR hetero L1/L2, LL homo/hetero L1) as arrays:
+  raw physical tags and the logical material / boundary ids their $PhysicalNames give
+  ("material_<k>", "boundary_<k>", io.py:165-189), which the Dirichlet labels refer to.
 
 Usage: python tests/golden/make_golden.py
 """
@@ -79,10 +83,42 @@ def histories() -> None:
     np.savez_compressed(OUT / "ssr_histories.npz", **out)
 
 
+def ll_histories() -> None:
+    """Load-factor (lambda_hist = t), omega and Umax histories of the reference's committed limit-load
+    runs (benchmarks/slope_stability_{2D_homo,3D_homo,3D_hetero}_LL, PETSc, 8 ranks)."""
+    out = {}
+    for tag, case in (("homo2d", "slope_stability_2D_homo_LL"), ("homo3d", "slope_stability_3D_homo_LL"),
+                      ("hetero3d", "slope_stability_3D_hetero_LL")):
+        d = np.load(REF / "benchmarks" / case / "artifacts" / "simulation" / "data" / "petsc_run.npz",
+                    allow_pickle=False)
+        for k in ("lambda_hist", "omega_hist", "Umax_hist"):
+            out[f"{tag}_{k}"] = np.asarray(d[k])
+    np.savez_compressed(OUT / "ll_histories.npz", **out)
+
+
+def meshes() -> None:
+    """The reference's Gmsh tet4 meshes as compact arrays (node coordinates, tet4 with physical
+    tags, boundary triangles with physical tags), read by this project's Gmsh 2.2 reader."""
+    for name, rel in (("ssr_hetero_L1_tet4", "3d_hetero_ssr/SSR_hetero_ada_L1.msh"),
+                      ("ssr_hetero_L2_tet4", "3d_hetero_ssr/SSR_hetero_ada_L2.msh"),
+                      ("ll_homo_L1_tet4", "3d_homo_ll/LL_homo_ada_L1.msh"),
+                      ("ll_hetero_L1_tet4", "3d_hetero_ll/LL_hetero_ada_L1.msh")):
+        path = REF / "meshes" / rel
+        coord, tet4, tet_phys, tri3, tri_phys = mesh.read_gmsh22(path)
+        names = mesh.read_gmsh_physical_names(path)
+        np.savez_compressed(OUT / f"{name}.npz", coord=coord, tet4=tet4.astype(np.int32),
+                            tet_phys=tet_phys.astype(np.int8), tri3=tri3.astype(np.int32),
+                            tri_phys=tri_phys.astype(np.int8),
+                            tet_label=mesh.logical_ids(tet_phys, names, 3, "material").astype(np.int8),
+                            tri_label=mesh.logical_ids(tri_phys, names, 2, "boundary").astype(np.int8))
+
+
 if __name__ == "__main__":
     if oracle.ref() is None:
         raise SystemExit("build the reference kernels first: make -C oracle ref")
     constitutive()
     assembly()
     histories()
+    ll_histories()
+    meshes()
     print("golden fixtures written to", OUT)

@@ -161,3 +161,23 @@ def test_node_slots_share_offsets_across_free_rows():
             np.testing.assert_array_equal(ns.off[a:b], exp)
             checked += 1
     assert checked > 0 and ns.max_len == int(np.diff(ip)[pat.row_free.astype(bool)].max())
+
+
+def test_gmsh_fixture_meshes_match_reference_unknowns():
+    """Free dof counts of the committed Gmsh meshes equal the reference runs' `unknowns`
+    (run_info.json of benchmarks/run_3D_hetero_SSR_capture: 80,362; slope_stability_3D_homo_LL:
+    73,459; slope_stability_3D_hetero_LL: 105,910 -- the last needs the physical-name mapping of
+    its boundary groups, whose tags are not their logical ids)."""
+    from slope_stability_petsc4py_amd import cases
+    assert int(cases.gmsh_fixture_mesh("ssr_hetero_L1_tet4.npz").q_mask.sum()) == 80362
+    assert int(cases.gmsh_fixture_mesh("ll_homo_L1_tet4.npz").q_mask.sum()) == 73459
+    assert int(cases.gmsh_fixture_mesh("ll_hetero_L1_tet4.npz").q_mask.sum()) == 105910
+
+
+def test_logical_ids_follow_physical_names():
+    names = {(2, 1): "boundary_0", (2, 2): "boundary_1", (2, 3): "boundary_3", (2, 4): "boundary_5",
+             (3, 1): "material_0"}
+    np.testing.assert_array_equal(mesh.logical_ids(np.array([1, 3, 4, 2]), names, 2, "boundary"), [0, 3, 5, 1])
+    np.testing.assert_array_equal(mesh.logical_ids(np.array([2, 7]), {}, 2, "boundary"), [2, 7])
+    with pytest.raises(ValueError):
+        mesh.logical_ids(np.array([9]), names, 2, "boundary")

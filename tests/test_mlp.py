@@ -59,3 +59,40 @@ def test_adam_first_step_is_sign_times_lr():
     g = np.array([0.3, -5.0])
     w2, m, v = O.adam(w, g, np.zeros(2), np.zeros(2), 1, lr=0.01)
     np.testing.assert_allclose(w2, w - 0.01 * np.sign(g), rtol=1e-6)
+
+
+def test_smooth_nodal_keeps_constants_and_damps_noise():
+    """The mesh smoother of the learned correction: a constant field is a fixed point, a random
+    field loses most of its point-to-point variation."""
+    import torch
+    from slope_stability_petsc4py_amd import mesh
+    from slope_stability_petsc4py_amd.learn import NodeInfo, smooth_nodal
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=5.0, nz=2)
+    elem = torch.from_numpy(np.ascontiguousarray(m.elem.T.astype(np.int64)))
+    cnt = np.bincount(m.elem.reshape(-1), minlength=m.n_nodes).astype(np.float32)
+    info = NodeInfo(None, None, None, 3, elem, torch.from_numpy((1.0 / np.maximum(cnt, 1))[:, None]))
+    c = torch.full((m.n_nodes, 3), 2.5)
+    assert torch.allclose(smooth_nodal(info, c, 4), c)
+    v = torch.from_numpy(np.random.default_rng(0).normal(size=(m.n_nodes, 3)).astype(np.float32))
+    sv = smooth_nodal(info, v, 4)
+    assert float(sv.std()) < 0.5 * float(v.std())
+    assert smooth_nodal(info, v, 0) is v
+
+
+def test_smooth_nodal_transpose_is_adjoint():
+    """<S v, g> == <v, S^T g> for the mesh smoother and its transpose (used for training through
+    the smoother)."""
+    import torch
+    from slope_stability_petsc4py_amd import mesh
+    from slope_stability_petsc4py_amd.learn import NodeInfo, smooth_nodal, smooth_nodal_T
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=5.0, nz=2)
+    elem = torch.from_numpy(np.ascontiguousarray(m.elem.T.astype(np.int64)))
+    cnt = np.bincount(m.elem.reshape(-1), minlength=m.n_nodes).astype(np.float64)
+    info = NodeInfo(None, None, None, 3, elem, torch.from_numpy((1.0 / np.maximum(cnt, 1))[:, None]))
+    rng = np.random.default_rng(1)
+    v = torch.from_numpy(rng.normal(size=(m.n_nodes, 3)))
+    g = torch.from_numpy(rng.normal(size=(m.n_nodes, 3)))
+    for k in (1, 3):
+        lhs = float((smooth_nodal(info, v, k) * g).sum())
+        rhs = float((v * smooth_nodal_T(info, g, k)).sum())
+        assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
