@@ -115,6 +115,18 @@ int ssx_dot(int64_t n, const double *x, const double *y, double *part, double *o
 int ssx_gemv_cols(int64_t n, int k, const double *V, int64_t ld, const double *h, double sign, double *x,
                   void *stream);
 
+/* Y_j = A X_j for k vectors stored as rows (X_j = X + j*ldx, Y_j = Y + j*ldy); the matrix is read
+ * once per 8 vectors.  Replaces the per-column PETSc MatMult of linear/orthogonalize.py:139-140. */
+int ssx_spmm_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int k,
+                 const double *X, int64_t ldx, double *Y, int64_t ldy, int lanes, void *stream);
+/* A-orthonormalisation of the deflation basis B (kmax rows of length n, row stride ldb, oldest
+ * first), newest column first with the drop rule |v . A b| <= tol (linear/orthogonalize.py:124-170):
+ * W[i], AW[i] = w and A w of column kmax-1-i, sgn[i] = sign of its A-norm (0 = dropped, W[i] = 0).
+ * work: scratch of at least 3*kmax + max(64*kmax, 1024) doubles.  No host round trip. */
+int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr,
+                        const int32_t *indices, const double *val, int lanes, double tol, double *W, double *AW,
+                        double *sgn, double *work, void *stream);
+
 /* Chebyshev/Jacobi smoother step: d = a*d + c*dinv.*r; x += d (replaces the Richardson/SOR and
  * Chebyshev/Jacobi smoother applies of linear/solver.py:155-162,417-436). */
 int ssx_cheb_update_f32(int64_t n, const float *dinv, const float *r, float c, float a, float *x, float *d,

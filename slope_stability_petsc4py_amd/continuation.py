@@ -96,12 +96,14 @@ def init_phase_ssr(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_dam
 
 def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, f, builder, solver, *,
-                              free_mask, dim, predictor=None, store_step_u=False, progress=None) -> SSRResult:
+                              free_mask, dim, predictor=None, store_step_u=False, progress=None,
+                              predictor_weights=(1.0, 0.5, 0.25)) -> SSRResult:
     """SSR indirect continuation; returns the lambda/omega/Umax histories (FoS = lambda_hist[-1]).
 
     ``predictor(ctx) -> U_guess or None`` is the learned-predictor slot: ctx holds the accepted
-    states, histories and the omega target; a returned guess is tried first and kept only if its
-    first nested-Newton residual passes the gate (below the secant's first residual)."""
+    states, histories and the omega target; a returned guess is blended with the secant guess at
+    the weights ``predictor_weights`` (1 = the guess itself), the blend with the smallest first
+    residual is tried first and kept only if that residual is below the secant's (the gate)."""
     t0 = perf_counter()
     stats = NewtonStats()
     scr = _Scratch(f.device)
@@ -140,16 +142,24 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         result = None
         gate_info = None
         if learned is not None:
-            # rescale to the omega constraint, then gate on the first residual vs the secant's
+            # rescale to the omega constraint, then gate on the first residual vs the secant's; the
+            # learned correction is also tried at reduced weight (a residual line search along
+            # learned - secant), so a correction that helps only in part is still used
             d = dot(f, learned, scr)
             if d != 0.0:
                 learned = learned * (omega_it / d)
             builder.reduction(lambda_value)
             F_sec = builder.build_F_reduced_vec(U_sec).clone()
             r_sec = float(torch.linalg.vector_norm(F_sec - f).item())
-            F_l = builder.build_F_reduced_vec(learned)
-            r_l = float(torch.linalg.vector_norm(F_l - f).item())
-            gate_info = {"secant_first_residual": r_sec, "learned_first_residual": r_l}
+            best, r_l, gamma_best = None, np.inf, 0.0
+            for gamma in predictor_weights:
+                cand = learned if gamma == 1.0 else U_sec + gamma * (learned - U_sec)
+                F_l = builder.build_F_reduced_vec(cand)
+                r_c = float(torch.linalg.vector_norm(F_l - f).item())
+                if r_c < r_l:
+                    best, r_l, gamma_best = cand, r_c, gamma
+            learned = best
+            gate_info = {"secant_first_residual": r_sec, "learned_first_residual": r_l, "learned_weight": gamma_best}
             if r_l < r_sec:
                 kind = "learned"
                 result = newton_ind_ssr(learned, omega_it, lambda_value, it_newt_max, it_damp_max, tol, r_min, f,

@@ -325,3 +325,210 @@ extern "C" int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const i
     SSX_CHECK_LAUNCH("ssx_galerkin_dense");
     return SSX_OK;
 }
+
+// ---------------------------------------------------------------------------------------------
+// Multi-vector SpMV and the A-orthonormalisation of the deflation basis.
+//
+// SpMM: Y_j = A X_j for k vectors stored as rows (X_j = X + j*ldx); a group of G lanes per row as
+// in the SpMV, KC vectors per pass, so the matrix is read once per KC vectors.
+//
+// A-orthonormalisation (linear/orthogonalize.py:124-170, the MATLAB A_orthogonalize): columns are
+// taken newest first; column i is v = b_i - sum_j sgn_j w_j (w_j . A b_i) with the raw product
+// A b_i, its signed A-norm is v . A b_i, and it is kept (w_i = v / sqrt|v . A b_i|, sgn_i = sign)
+// when |v . A b_i| > tol, else dropped (w_i = 0, sgn_i = 0).  Here every A b_i comes from one SpMM
+// pass before the sweep, the per-column work is four launches with no host round trip, and the
+// scaling of w_i is carried as scale_i = 1/sqrt|nv_i| until one final pass (the stored rows are
+// unscaled during the sweep, so the coefficient of row j is scale_j^2 sgn_j (row_j . A b_i)).
+// A W is formed alongside by the same recurrence from the raw products (A w_i = scale_i (A b_i -
+// sum_j h_j A row_j)), which replaces a second SpMV per kept column.
+// ---------------------------------------------------------------------------------------------
+namespace {
+
+constexpr int kSpmmKC = 8;
+
+template <int G>
+__global__ void __launch_bounds__(256) k_spmm_f64(int64_t n_rows, const int32_t *__restrict__ indptr,
+                                                  const int32_t *__restrict__ indices, const double *__restrict__ val,
+                                                  int kc, const double *__restrict__ X, int64_t ldx,
+                                                  double *__restrict__ Y, int64_t ldy) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = tid / G;
+    const int lane = (int)(tid % G);
+    double acc[kSpmmKC];
+#pragma unroll
+    for (int j = 0; j < kSpmmKC; ++j) acc[j] = 0.0;
+    if (row < n_rows) {
+        const int32_t k1 = indptr[row + 1];
+        for (int32_t k = indptr[row] + lane; k < k1; k += G) {
+            const double a = val[k];
+            const int64_t c = indices[k];
+#pragma unroll
+            for (int j = 0; j < kSpmmKC; ++j)
+                if (j < kc) acc[j] += a * X[(int64_t)j * ldx + c];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < kSpmmKC; ++j)
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) acc[j] += __shfl_down(acc[j], off, G);
+    if (row < n_rows && lane == 0) {
+#pragma unroll
+        for (int j = 0; j < kSpmmKC; ++j)
+            if (j < kc) Y[(int64_t)j * ldy + row] = acc[j];
+    }
+}
+
+int launch_spmm(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int k,
+                const double *X, int64_t ldx, double *Y, int64_t ldy, int lanes, hipStream_t st) {
+    const dim3 grid(ssx::grid_for(n_rows * lanes, 256)), block(256);
+    for (int j0 = 0; j0 < k; j0 += kSpmmKC) {
+        const int kc = k - j0 < kSpmmKC ? k - j0 : kSpmmKC;
+        const double *Xj = X + (int64_t)j0 * ldx;
+        double *Yj = Y + (int64_t)j0 * ldy;
+        switch (lanes) {
+            case 1: hipLaunchKernelGGL((k_spmm_f64<1>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            case 2: hipLaunchKernelGGL((k_spmm_f64<2>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            case 4: hipLaunchKernelGGL((k_spmm_f64<4>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            case 8: hipLaunchKernelGGL((k_spmm_f64<8>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            case 16: hipLaunchKernelGGL((k_spmm_f64<16>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            case 32: hipLaunchKernelGGL((k_spmm_f64<32>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            case 64: hipLaunchKernelGGL((k_spmm_f64<64>), grid, block, 0, st, n_rows, indptr, indices, val, kc, Xj, ldx, Yj, ldy); break;
+            default: ssx::set_error("ssx_spmm_f64: lanes per row must be a power of two <= 64"); return SSX_EINVAL;
+        }
+    }
+    SSX_CHECK_LAUNCH("ssx_spmm_f64");
+    return SSX_OK;
+}
+
+// W[i] = B[kmax - 1 - i] (newest column first)
+__global__ void k_copy_rows_rev(int64_t n, int kmax, const double *__restrict__ B, int64_t ldb, double *__restrict__ W) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kmax) return;
+    const int64_t i = t / n, r = t - i * n;
+    W[t] = B[(kmax - 1 - i) * ldb + r];
+}
+
+// h[j] = scale_j^2 sgn_j sum_c part[j*n_chunk + c]
+__global__ void k_aorth_coeff(int k, int n_chunk, const double *__restrict__ part, const double *__restrict__ scale,
+                              const double *__restrict__ sgn, double *__restrict__ h) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= k) return;
+    double acc = 0.0;
+    for (int c = 0; c < n_chunk; ++c) acc += part[(int64_t)j * n_chunk + c];
+    h[j] = scale[j] * scale[j] * sgn[j] * acc;
+}
+
+// row i of W and AW: v = W_i - sum_j h_j W_j, aw = AW_i - sum_j h_j AW_j (AW_i holds A b_i on entry);
+// block partials of v . (A b_i) into part
+constexpr int kAorthBlocks = 1024;
+
+__global__ void __launch_bounds__(256) k_aorth_update(int64_t n, int i, double *__restrict__ W,
+                                                      double *__restrict__ AW, const double *__restrict__ h,
+                                                      double *__restrict__ part) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    double *wi = W + (int64_t)i * n;
+    double *awi = AW + (int64_t)i * n;
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double v = wi[r];
+        const double ab = awi[r];
+        double aw = ab;
+        for (int j = 0; j < i; ++j) {
+            const double hj = h[j];
+            v -= hj * W[(int64_t)j * n + r];
+            aw -= hj * AW[(int64_t)j * n + r];
+        }
+        wi[r] = v;
+        awi[r] = aw;
+        acc += v * ab;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// nv_i = sum of the partials (fixed order); keep test, sign and scale of column i
+__global__ void __launch_bounds__(256) k_aorth_finish(int nb, const double *__restrict__ part, int i, double tol,
+                                                      double *__restrict__ nv, double *__restrict__ sgn,
+                                                      double *__restrict__ scale) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += part[b];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double v = red[0];
+        const double a = fabs(v);
+        const bool keep = a > tol;
+        nv[i] = v;
+        sgn[i] = keep ? (v > 0.0 ? 1.0 : -1.0) : 0.0;
+        scale[i] = keep ? 1.0 / sqrt(a) : 0.0;
+    }
+}
+
+__global__ void k_aorth_scale(int64_t n, int kmax, const double *__restrict__ scale, double *__restrict__ W,
+                              double *__restrict__ AW) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * kmax) return;
+    const double s = scale[t / n];
+    W[t] *= s;
+    AW[t] *= s;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ssx_spmm_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int k,
+                 const double *X, int64_t ldx, double *Y, int64_t ldy, int lanes, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && k >= 0 && ldx >= n_rows && ldy >= n_rows, "ssx_spmm_f64: bad sizes");
+    if (n_rows == 0 || k == 0) return SSX_OK;
+    SSX_REQUIRE(indptr && indices && val && X && Y, "ssx_spmm_f64: null pointer");
+    return launch_spmm(n_rows, indptr, indices, val, k, X, ldx, Y, ldy, lanes, ssx::as_stream(stream));
+}
+
+int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr,
+                        const int32_t *indices, const double *val, int lanes, double tol, double *W, double *AW,
+                        double *sgn, double *work, void *stream) {
+    SSX_REQUIRE(n > 0 && kmax >= 0 && ldb >= n, "ssx_a_orthogonalize: bad sizes");
+    SSX_REQUIRE(kmax <= 4096, "ssx_a_orthogonalize: at most 4096 basis columns");
+    if (kmax == 0) return SSX_OK;
+    SSX_REQUIRE(B && indptr && indices && val && W && AW && sgn && work, "ssx_a_orthogonalize: null pointer");
+    hipStream_t st = ssx::as_stream(stream);
+    double *h = work;                 // kmax
+    double *scale = work + kmax;      // kmax
+    double *nv = work + 2 * kmax;     // kmax
+    double *part = work + 3 * kmax;   // max(kmax * kDotMaxChunks, kAorthBlocks)
+    const int64_t total = n * (int64_t)kmax;
+    hipLaunchKernelGGL(k_copy_rows_rev, dim3(ssx::grid_for(total, 256)), dim3(256), 0, st, n, kmax, B, ldb, W);
+    SSX_CHECK_LAUNCH("ssx_a_orthogonalize");
+    const int rc = launch_spmm(n, indptr, indices, val, kmax, W, n, AW, n, lanes, st);
+    if (rc != SSX_OK) return rc;
+    int64_t n_chunk = (n + kDotChunk - 1) / kDotChunk;
+    if (n_chunk > kDotMaxChunks) n_chunk = kDotMaxChunks;
+    const int64_t chunk = (n + n_chunk - 1) / n_chunk;
+    const int nb = (int)(n / 256 + 1 < kAorthBlocks ? n / 256 + 1 : kAorthBlocks);
+    for (int i = 0; i < kmax; ++i) {
+        if (i > 0) {
+            hipLaunchKernelGGL(k_dot_cols_part, dim3((unsigned)n_chunk, (unsigned)i), dim3(256), 0, st, n, W, n,
+                               AW + (int64_t)i * n, chunk, part);
+            hipLaunchKernelGGL(k_aorth_coeff, dim3((unsigned)((i + 63) / 64)), dim3(64), 0, st, i, (int)n_chunk, part,
+                               scale, sgn, h);
+        }
+        hipLaunchKernelGGL(k_aorth_update, dim3((unsigned)nb), dim3(256), 0, st, n, i, W, AW, h, part);
+        hipLaunchKernelGGL(k_aorth_finish, dim3(1), dim3(256), 0, st, nb, part, i, tol, nv, sgn, scale);
+    }
+    hipLaunchKernelGGL(k_aorth_scale, dim3(ssx::grid_for(total, 256)), dim3(256), 0, st, n, kmax, scale, W, AW);
+    SSX_CHECK_LAUNCH("ssx_a_orthogonalize");
+    return SSX_OK;
+}
+
+}  // extern "C"

@@ -613,9 +613,10 @@ class DeflatedFGMRES:
         """A-orthonormalise the basis newest-first, dropping columns with |v^T A v| <= eps
         (linear/orthogonalize.py:49-170).
 
-        Runs without host round trips per column: the coefficients, the A-norm test and the
-        scaling stay on the device; a dropped column becomes a zero row with sign 0 (so it takes
-        no part in later projections), and the kept rows are compacted after one host read."""
+        One C-ABI call (`ssx_a_orthogonalize`): all raw products A b come from one multi-vector
+        SpMV, the coefficients, the A-norm test and the scaling stay on the device, a dropped
+        column becomes a zero row with sign 0 (so it takes no part in later projections), and A W
+        is carried by the same recurrence; the kept rows are compacted after one host read."""
         self._A = A
         B = self.deflation_basis
         if B is None or B.shape[0] == 0:
@@ -623,37 +624,24 @@ class DeflatedFGMRES:
             self._AW = None
             return
         dev = B.device
+        B = B.contiguous()
         kmax, n = B.shape
+        if n != A.n_rows:
+            raise ValueError(f"A_orthogonalize: basis length {n} != operator rows {A.n_rows}")
         Wbuf = torch.empty((kmax, n), dtype=torch.float64, device=dev)
-        av = torch.empty(n, dtype=torch.float64, device=dev)
-        coeff = torch.empty(kmax, dtype=torch.float64, device=dev)
-        sgn = torch.zeros(kmax, dtype=torch.float64, device=dev)
-        nv = torch.empty(kmax, dtype=torch.float64, device=dev)
-        scr = self._scr if self._scr is not None else _Scratch(dev)
-        for i in range(kmax):
-            v = Wbuf[i]
-            v.copy_(B[kmax - 1 - i])
-            spmv(A, v, av)
-            if i:
-                _lib.call("ssx_dot_cols", n, i, ptr(Wbuf), n, ptr(av), ptr(scr.cols_for(i)), ptr(coeff), _st())
-                coeff[:i].mul_(sgn[:i])
-                _lib.call("ssx_gemv_cols", n, i, ptr(Wbuf), n, ptr(coeff), -1.0, ptr(v), _st())
-            _lib.call("ssx_dot", n, ptr(v), ptr(av), ptr(scr.part), ptr(nv[i:]), _st())
-            a = nv[i].abs()
-            keep = a > self.tolerance_deflation_basis
-            v.mul_(torch.where(keep, torch.rsqrt(torch.where(keep, a, torch.ones_like(a))), torch.zeros_like(a)))
-            sgn[i] = torch.where(keep, torch.sign(nv[i]), torch.zeros_like(a))
+        AWbuf = torch.empty((kmax, n), dtype=torch.float64, device=dev)
+        sgn = torch.empty(kmax, dtype=torch.float64, device=dev)
+        work = torch.empty(3 * kmax + max(64 * kmax, 1024), dtype=torch.float64, device=dev)
+        _lib.call("ssx_a_orthogonalize", n, kmax, ptr(B), n, ptr(A.indptr), ptr(A.indices), ptr(A.values), A.lanes,
+                  float(self.tolerance_deflation_basis), ptr(Wbuf), ptr(AWbuf), ptr(sgn), ptr(work), _st())
         kept = torch.nonzero(sgn != 0).flatten().cpu().numpy()   # the one host read
         if kept.size == 0:
             self._W = None
             self._AW = None
             return
         order = torch.from_numpy(kept[::-1].copy()).to(dev)     # oldest first
-        W = Wbuf.index_select(0, order).contiguous()
-        AW = torch.empty_like(W)
-        for j in range(W.shape[0]):
-            spmv(A, W[j], AW[j])
-        self._W, self._AW = W, AW
+        self._W = Wbuf.index_select(0, order).contiguous()
+        self._AW = AWbuf.index_select(0, order).contiguous()
 
     def solve(self, A: DeviceCSR, b: torch.Tensor, full_rhs=None, local_rhs=None, free_indices=None):
         t0 = perf_counter()

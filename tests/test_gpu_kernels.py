@@ -206,6 +206,32 @@ def test_spmv_vs_oracle(gpu, asm3d):
         assert _rel(K.matvec(torch.from_numpy(x).to(dev)).cpu().numpy(), yo) < 1e-13
 
 
+@pytest.mark.parametrize("k", [1, 8, 13])
+def test_spmm_vs_oracle(gpu, asm3d, k):
+    """Multi-vector SpMV (k vectors as rows, padded row stride, passes of 8) == k oracle SpMVs."""
+    from slope_stability_petsc4py_amd.device import ptr
+    print(f"case: spmm k={k}", flush=True)
+    m, geom, pat = asm3d
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    rng = np.random.default_rng(15 + k)
+    val = rng.normal(size=pat.nnz)
+    K = dp.new_csr()
+    K.values.copy_(torch.from_numpy(val))
+    n = pat.n_rows
+    ld = n + 5
+    X = rng.normal(size=(k, ld))
+    Xd = torch.from_numpy(X).to(dev)
+    Yd = torch.full((k, ld), 7.0, dtype=torch.float64, device=dev)
+    for lanes in (4, 16):
+        _lib.call("ssx_spmm_f64", n, ptr(K.indptr), ptr(K.indices), ptr(K.values), k, ptr(Xd), ld, ptr(Yd), ld,
+                  lanes, 0)
+        Y = Yd.cpu().numpy()
+        for j in range(k):
+            assert _rel(Y[j, :n], oracle.spmv(pat.indptr, pat.indices, val, X[j, :n])) < 1e-13
+        assert np.all(Y[:, n:] == 7.0)
+
+
 def test_vector_kernels(gpu):
     dev = torch.device("cuda", 0)
     rng = np.random.default_rng(8)
@@ -380,3 +406,42 @@ def test_node_gather_kr_fold_vs_oracle(gpu, elem_type):
     exp0 = Kt.copy()
     exp0[pat.constrained_diag_positions] = 1.0
     assert _rel(K2.cpu().numpy(), exp0) < 1e-12
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_p2_element_kernel_variants_agree(gpu, variant):
+    """The direct and the pipelined element-matrix kernels give the oracle's regularised element
+    matrices on an odd element count (the persistent grid's last partial sweep)."""
+    print(f"case: P2 element kernel variant {variant}", flush=True)
+    from slope_stability_petsc4py_amd import _lib
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=7.5, nz=3)
+    geom = fem.geometry(m.coord, m.elem, "P2")
+    n_e = geom.n_elem - 1 if geom.n_elem % 2 == 0 else geom.n_elem   # odd count
+    dev = gpu
+    rng = np.random.default_rng(43)
+    A = rng.normal(size=(geom.n_int, 6, 6))
+    DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
+    G = rng.uniform(1e3, 2e4, geom.n_int)
+    lam = rng.uniform(1e3, 3e4, geom.n_int)
+    vol = np.array([1, 1, 1, 0, 0, 0.0])
+    idn = np.diag([1, 1, 1, 0.5, 0.5, 0.5])
+    Del = (lam[:, None, None] * np.outer(vol, vol)[None] + 2 * G[:, None, None] * idn[None]).reshape(-1, 36)
+    r = 0.25
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    n_int = n_e * geom.n_q
+    dphi = np.ascontiguousarray(geom.dphi[:, :n_int, :])     # the first n_e elements only
+    wt, DS, G, lam, Del = geom.weight[:n_int], DS[:n_int], G[:n_int], lam[:n_int], Del[:n_int]
+    ke = torch.full((n_e, 30, 30), np.nan, dtype=torch.float64, device=dev)
+    args = [t(dphi), t(DS), t(wt), t(G), t(lam)]
+    _lib.call("ssx_set_element_kernel", variant, 16)
+    try:
+        _lib.call("ssx_element_matrices_3d", 10, int(args[0].data_ptr()), int(args[1].data_ptr()),
+                  int(args[2].data_ptr()), n_e, geom.n_q, r, int(args[3].data_ptr()), int(args[4].data_ptr()),
+                  int(ke.data_ptr()), 0)
+        torch.cuda.synchronize()
+    finally:
+        _lib.call("ssx_set_element_kernel", 1, 16)
+    exp = oracle.element_matrices_3d(dphi, (1 - r) * DS + r * Del, wt, n_e, geom.n_q)
+    got = ke.cpu().numpy()
+    assert np.isfinite(got).all()
+    assert _rel(got, exp) < 1e-12

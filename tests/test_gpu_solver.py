@@ -98,3 +98,25 @@ def test_a_orthogonalize_matches_oracle_and_drops_dependent_columns(gpu, elastic
     G = W @ (Ks @ W.T)
     assert np.abs(G - np.eye(5)).max() < 1e-8
     np.testing.assert_allclose(solver._AW.cpu().numpy(), (Ks @ W.T).T, rtol=0, atol=1e-9 * np.abs(G).max() * 1e3)
+
+
+@pytest.mark.parametrize("case_name", ["homo2d_ll", "homo3d_ll_L1"])
+def test_ll_matches_reference_limit_load(gpu, case_name):
+    """Limit-load continuation (continuation/limit_load.py) on the reference's LL benchmarks
+    (benchmarks/slope_stability_2D_homo_LL, slope_stability_3D_homo_LL on LL_homo_ada_L1): the load
+    factor reached at the end of the branch within 1e-3 (relative) of the reference's last entry,
+    and the elastic start on the reference's line omega = c t (c = f.K^-1 f; the reference's first
+    omega target comes from an elastic solve at linear tolerance 1e-1, so only the ratio is fixed)."""
+    print(f"case: LL {case_name}", flush=True)
+    h = np.load(GOLD / "ll_histories.npz")
+    key = "homo2d" if case_name == "homo2d_ll" else "homo3d"
+    ref_t, ref_om = h[key + "_lambda_hist"], h[key + "_omega_hist"]
+    case = cases.homo2d_ll() if case_name == "homo2d_ll" else cases.homo3d_ll_L1()
+    s = cases.make_solver(case, precision="fp64" if case_name == "homo2d_ll" else "fp32")
+    res = cases.run_ll(s)
+    t = np.asarray(res.t_hist)
+    om = np.asarray(res.omega_hist)
+    print("t", t[-1], "ref", ref_t[-1], "steps", len(t), "wall", res.wall_time, flush=True)
+    assert abs(t[-1] - ref_t[-1]) <= 1e-3 * ref_t[-1]
+    assert abs(om[1] / t[1] - ref_om[1] / ref_t[1]) <= 1e-3 * ref_om[1] / ref_t[1]
+    assert np.all(np.diff(t[:-1]) > 0)
