@@ -216,6 +216,16 @@ int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *ind
                        const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c, double *Ac,
                        void *stream);
 
+/* Coarse solve y = Ainv x of the p-multigrid with the explicit inverse of the symmetric coarse
+ * operator kept as the upper triangle of 64 x 64 tiles (half the bytes of a dense product per
+ * apply; deterministic).  Replaces the coarse solve of the manual V-cycle (linear/solver.py:97-160,
+ * Hypre there).  Sizes: tiles = ceil(n/64)(ceil(n/64)+1)/2 * 4096 elements, part = ceil(n/64)^2 * 64. */
+int ssx_sym_tiles_sizes(int64_t n, int64_t *tile_elems, int64_t *part_elems);
+/* Pack a symmetric n x n matrix (FP64 if a_f64, else FP32) into tiles (FP64 if t_f64, else FP32). */
+int ssx_sym_tiles_pack(int64_t n, const void *A, int a_f64, void *tiles, int t_f64, void *stream);
+int ssx_sym_tiles_gemv_f32(int64_t n, const float *tiles, const float *x, float *part, float *y, void *stream);
+int ssx_sym_tiles_gemv_f64(int64_t n, const double *tiles, const double *x, double *part, double *y, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

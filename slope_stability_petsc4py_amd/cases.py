@@ -139,21 +139,22 @@ class Solver:
 
 
 def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_refresh: int = 0,
-                coarse_precision: str | None = None) -> Solver:
+                coarse_precision: str | None = None, refresh_iterations: int = 25) -> Solver:
     """precision 'fp64': FP64 preconditioner; 'fp32': FP32 tangent shadow + FP32 preconditioner
     (the outer FGMRES, residuals and return mapping stay FP64).  `coarse_refresh`: the p-multigrid
     coarse operator is rebuilt every that many preconditioner setups (1 = every Newton iteration),
-    or, with 0 (default), when a solve needs more than 15 FGMRES iterations."""
+    or, with 0 (default), when a solve needs more than `refresh_iterations` FGMRES iterations."""
     use32 = precision == "fp32"
     op = ConstitutiveOperator.from_mesh(case.mesh, case.materials, Davis_type=case.davis, fp32_shadow=use32)
     f_field = op.gravity_load(case.mesh)
     f = to_dev(f_field.T.reshape(-1).copy())
     free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
     if pc == "pmg" and case.mesh.elem_type == "P4":
-        prec = PMultigridP4(case.mesh, op, use32=use32, coarse_refresh=coarse_refresh)
+        prec = PMultigridP4(case.mesh, op, use32=use32, coarse_refresh=coarse_refresh,
+                            refresh_iterations=refresh_iterations)
     elif pc == "pmg":
         prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32, coarse_refresh=coarse_refresh,
-                          coarse_precision=coarse_precision)
+                          coarse_precision=coarse_precision, refresh_iterations=refresh_iterations)
     else:
         prec = ChebyshevJacobi(use32=use32)
     lin = DeflatedFGMRES(prec, tolerance=case.params.linear_tol, max_iterations=case.params.linear_maxit)

@@ -101,9 +101,10 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
     """SSR indirect continuation; returns the lambda/omega/Umax histories (FoS = lambda_hist[-1]).
 
     ``predictor(ctx) -> U_guess or None`` is the learned-predictor slot: ctx holds the accepted
-    states, histories and the omega target; a returned guess is blended with the secant guess at
-    the weights ``predictor_weights`` (1 = the guess itself), the blend with the smallest first
-    residual is tried first and kept only if that residual is below the secant's (the gate)."""
+    states, histories and the omega target; it returns U_guess or (U_guess, lambda_guess).  The
+    guess is blended with the secant guess at the weights ``predictor_weights`` (1 = the guess
+    itself), the blend with the smallest first residual (at lambda_guess, default the current
+    lambda) is tried first and kept only if that residual is below the secant's (the gate)."""
     t0 = perf_counter()
     stats = NewtonStats()
     scr = _Scratch(f.device)
@@ -141,6 +142,9 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         newton_it_before = stats.newton_iterations
         result = None
         gate_info = None
+        lam_guess = lambda_value
+        if isinstance(learned, tuple):
+            learned, lam_guess = learned[0], float(learned[1])
         if learned is not None:
             # rescale to the omega constraint, then gate on the first residual vs the secant's; the
             # learned correction is also tried at reduced weight (a residual line search along
@@ -151,6 +155,8 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
             builder.reduction(lambda_value)
             F_sec = builder.build_F_reduced_vec(U_sec).clone()
             r_sec = float(torch.linalg.vector_norm(F_sec - f).item())
+            if lam_guess != lambda_value:
+                builder.reduction(lam_guess)
             best, r_l, gamma_best = None, np.inf, 0.0
             for gamma in predictor_weights:
                 cand = learned if gamma == 1.0 else U_sec + gamma * (learned - U_sec)
@@ -162,7 +168,7 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
             gate_info = {"secant_first_residual": r_sec, "learned_first_residual": r_l, "learned_weight": gamma_best}
             if r_l < r_sec:
                 kind = "learned"
-                result = newton_ind_ssr(learned, omega_it, lambda_value, it_newt_max, it_damp_max, tol, r_min, f,
+                result = newton_ind_ssr(learned, omega_it, lam_guess, it_newt_max, it_damp_max, tol, r_min, f,
                                         builder, solver, free_mask=free_mask, stats=stats)
                 if result[2] != 0:
                     solver.restore_deflation_basis(basis_snap)

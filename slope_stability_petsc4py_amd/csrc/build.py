@@ -23,6 +23,7 @@ SOURCES = [
     "constitutive.hip",
     "assembly.hip",
     "sparse.hip",
+    "coarse.hip",
     "p4_mfma.hip",
     "mlp.hip",
     "elem_blocks.hip",

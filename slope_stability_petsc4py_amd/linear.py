@@ -67,6 +67,14 @@ def axpby(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> None:
         _lib.call("ssx_axpby_f32", y.numel(), float(a), ptr(x), float(b), ptr(y), _st())
 
 
+def sym_tile_sizes(n: int) -> tuple[int, int]:
+    """Element counts of the symmetric tile store of an n x n matrix and of its apply's partials."""
+    import ctypes
+    out = (ctypes.c_int64 * 2)()
+    _lib.call("ssx_sym_tiles_sizes", int(n), ctypes.addressof(out), ctypes.addressof(out) + 8)
+    return int(out[0]), int(out[1])
+
+
 def spmv(A: DeviceCSR, x: torch.Tensor, y: torch.Tensor, alpha=1.0, beta=0.0, use32: bool = False) -> None:
     if use32:
         _lib.call("ssx_spmv_f32", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values32), ptr(x), ptr(y),
@@ -325,7 +333,7 @@ class PMultigridP4:
     (Jacobi) smoothing on the P4 and P2 levels, FP32 or FP64."""
 
     def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 0):
+                 coarse_refresh: int = 0, refresh_iterations: int = 25):
         from .device import DevicePattern
         from .fem import geometry, quadrature_3d
         from .pattern import build_owned_pattern
@@ -343,7 +351,7 @@ class PMultigridP4:
         # per-point shear and lame into its element matrices
         self.A2 = self.pat2.new_csr(with32=True)
         self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac,
-                                 coarse_refresh=coarse_refresh)
+                                 coarse_refresh=coarse_refresh, refresh_iterations=refresh_iterations)
         self.note_iterations = self.coarse.note_iterations
         self.timings = self.coarse.timings
         self.level = None
@@ -396,7 +404,7 @@ class PMultigrid:
     only costs iterations, never correctness)."""
 
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 15):
+                 coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 25):
         self.use32 = use32
         self.degree = degree
         self.lo_frac = lo_frac
@@ -413,11 +421,21 @@ class PMultigrid:
                         "coarse_fp64_fallbacks": 0}
         self._setups = 0
         p_ptr, p_idx, p_w, n_c, c_free = p2_to_p1_prolongation(elem, q_mask)
-        self.n_c = n_c
-        self.c_free = c_free
+        # the coarse space is numbered over the free coarse dofs only: no column of P touches a
+        # constrained coarse dof, so the dense coarse system (factorised and inverted at each
+        # rebuild, applied every Krylov iteration) carries no identity block
+        keep = np.flatnonzero(c_free)
+        remap = np.full(n_c, -1, dtype=np.int64)
+        remap[keep] = np.arange(keep.size)
+        p_idx = remap[p_idx]
+        if p_idx.size and p_idx.min() < 0:
+            raise AssertionError("P2 -> P1 interpolation references a constrained coarse dof")
+        self.n_c = int(keep.size)
+        self.c_free = np.ones(self.n_c, dtype=bool)
         self._p_host = (p_ptr, p_idx, p_w)
         self.level = None
-        self.Ainv = None
+        self.Ainv = None          # packed symmetric tiles of the coarse inverse
+        self._tile_part = None
         self._dev_ready = False
 
     def _to_device(self, dev, n_f):
@@ -493,9 +511,16 @@ class PMultigrid:
         torch.cuda.synchronize()
         t2 = perf_counter()
         tm["factor"] += t2 - t1
-        Ainv = torch.cholesky_inverse(L)
+        Ainv = torch.cholesky_inverse(L).contiguous()
         del L
-        self.Ainv = Ainv.to(torch.float32) if self.use32 else Ainv.to(torch.float64)
+        # the inverse is kept as the upper triangle of 64 x 64 tiles in the level precision
+        dt = torch.float32 if self.use32 else torch.float64
+        te, pe = sym_tile_sizes(self.n_c)
+        if self._tile_part is None or self._tile_part.dtype != dt:
+            self._tile_part = torch.empty(pe, dtype=dt, device=dev)
+        self.Ainv = torch.empty(te, dtype=dt, device=dev)
+        _lib.call("ssx_sym_tiles_pack", self.n_c, ptr(Ainv), int(Ainv.dtype == torch.float64), ptr(self.Ainv),
+                  int(dt == torch.float64), _st())
         del Ainv
         torch.cuda.synchronize()
         tm["inverse"] += perf_counter() - t2
@@ -509,7 +534,8 @@ class PMultigrid:
         axpby(1.0, b, 0.0, res)
         spmv(Lv.A, x, res, alpha=-1.0, beta=1.0, use32=self.use32)          # res = b - A x
         spmv(self.PT, res, self.rc, use32=self.use32)                          # restrict
-        torch.mv(self.Ainv, self.rc, out=self.xc)                              # coarse solve
+        _lib.call("ssx_sym_tiles_gemv_f32" if self.use32 else "ssx_sym_tiles_gemv_f64", self.n_c,  # coarse solve
+                  ptr(self.Ainv), ptr(self.rc), ptr(self._tile_part), ptr(self.xc), _st())
         spmv(self.P, self.xc, x, alpha=1.0, beta=1.0, use32=self.use32)       # prolong + correct
         Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=False)          # post-smooth
 

@@ -32,7 +32,9 @@ def main(argv=None):
     ap.add_argument("--omega-max", type=float, default=None, help="override omega_max_stop")
     ap.add_argument("--coarse-refresh", type=int, default=0,
                     help="rebuild the multigrid coarse operator every N preconditioner setups (0: when a solve "
-                         "needs more than 15 iterations)")
+                         "needs more than --refresh-iterations iterations)")
+    ap.add_argument("--refresh-iterations", type=int, default=25,
+                    help="FGMRES iterations of one solve above which the coarse operator is rebuilt")
     ap.add_argument("--coarse-precision", default=None, choices=["fp32", "fp64"],
                     help="coarse factorisation precision (default: the preconditioner precision)")
     ap.add_argument("--verbose", action="store_true")
@@ -57,7 +59,7 @@ def main(argv=None):
     if a.omega_max:
         case.params.omega_max_stop = a.omega_max
     s = cases.make_solver(case, precision=a.precision, pc=a.pc, coarse_refresh=a.coarse_refresh,
-                          coarse_precision=a.coarse_precision)
+                          coarse_precision=a.coarse_precision, refresh_iterations=a.refresh_iterations)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
 

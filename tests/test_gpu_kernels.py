@@ -445,3 +445,37 @@ def test_p2_element_kernel_variants_agree(gpu, variant):
     got = ke.cpu().numpy()
     assert np.isfinite(got).all()
     assert _rel(got, exp) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 64, 200, 1000])
+@pytest.mark.parametrize("f64", [False, True])
+def test_sym_tiles_gemv_vs_numpy(gpu, n, f64):
+    """Coarse solve with the symmetric tile store == the dense product with the full matrix."""
+    from slope_stability_petsc4py_amd.device import ptr
+    from slope_stability_petsc4py_amd.linear import sym_tile_sizes
+    print(f"case: sym tiles gemv n={n} f64={f64}", flush=True)
+    rng = np.random.default_rng(n)
+    M = rng.normal(size=(n, n))
+    A = M + M.T
+    x = rng.normal(size=n)
+    dt = torch.float64 if f64 else torch.float32
+    te, pe = sym_tile_sizes(n)
+    nt = (n + 63) // 64
+    assert te == nt * (nt + 1) // 2 * 4096 and pe == nt * nt * 64
+    Ad = torch.from_numpy(A).to(gpu)              # FP64 source, packed into the level precision
+    tiles = torch.full((te,), np.nan, dtype=dt, device=gpu)
+    part = torch.empty(pe, dtype=dt, device=gpu)
+    y = torch.full((n,), np.nan, dtype=dt, device=gpu)
+    _lib.call("ssx_sym_tiles_pack", n, ptr(Ad), 1, ptr(tiles), int(f64), 0)
+    xd = torch.from_numpy(x).to(dt).to(gpu)
+    _lib.call("ssx_sym_tiles_gemv_f64" if f64 else "ssx_sym_tiles_gemv_f32", n, ptr(tiles), ptr(xd), ptr(part),
+              ptr(y), 0)
+    got = y.cpu().numpy().astype(np.float64)
+    exp = A @ x
+    assert _rel(got, exp) < (1e-13 if f64 else 2e-6)
+    # FP32 source packed into FP32 tiles gives the same product
+    if not f64:
+        A32 = torch.from_numpy(A.astype(np.float32)).to(gpu)
+        _lib.call("ssx_sym_tiles_pack", n, ptr(A32), 0, ptr(tiles), 0, 0)
+        _lib.call("ssx_sym_tiles_gemv_f32", n, ptr(tiles), ptr(xd), ptr(part), ptr(y), 0)
+        assert _rel(y.cpu().numpy().astype(np.float64), exp) < 2e-6
