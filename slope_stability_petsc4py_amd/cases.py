@@ -23,6 +23,7 @@ from .constitutive import ConstitutiveOperator
 from .continuation import ll_indirect_continuation, ssr_indirect_continuation
 from .device import to_dev
 from .linear import ChebyshevJacobi, DeflatedFGMRES, PMultigrid, PMultigridP4
+from .loads import strip_footing_load
 from .materials import MaterialSpec
 from .workloads import HETERO_LAYERS, HOMO
 
@@ -60,6 +61,10 @@ class Case:
     params: SSRParams
     davis: str = "B"
     extra: dict = field(default_factory=dict)
+    # limit load under a strip-footing pressure: (x_min, x_max, q) of the loaded top-surface strip;
+    # the continuation load f is then the footing load, and gravity (if kept) a constant load
+    footing: tuple | None = None
+    gravity: bool = True
 
 
 def homo2d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.0) -> Case:
@@ -121,6 +126,38 @@ def hetero3d_ll_L1(elem_type: str = "P2") -> Case:
                       _ll_params(0.5, 8.0e7, 200))
 
 
+def prandtl_nc(phi_deg: float) -> float:
+    """Prandtl's bearing-capacity factor N_c = (N_q - 1) cot(phi), N_q = e^(pi tan phi) tan^2(45 + phi/2):
+    the limit pressure of a strip footing on a weightless associated Mohr-Coulomb half-space is c N_c."""
+    ph = np.deg2rad(phi_deg)
+    if ph == 0.0:
+        return 2.0 + np.pi
+    nq = np.exp(np.pi * np.tan(ph)) * np.tan(np.pi / 4 + ph / 2) ** 2
+    return float((nq - 1.0) / np.tan(ph))
+
+
+def footing2d_prandtl(h: float = 0.25, phi: float = 10.0, c: float = 10.0) -> Case:
+    """Strip footing (half width 1 m at the symmetry plane x = 0) on a weightless, associated
+    (psi = phi) Mohr-Coulomb layer, plane strain, P2 triangles; the footing pressure is q = c, so
+    the limit load factor approaches N_c(phi) (Prandtl) as the mesh is refined.  The 16 x 9 m
+    domain holds the Prandtl mechanism (a few metres wide and deep) well inside."""
+    m = M.slope_2d_p2(h=h, x1=12.0, x2=1.0, x3=3.0, y1=8.0, y2=1.0)
+    mat = MaterialSpec(c, phi, phi, 40000.0, 0.30, 0.0, 0.0)
+    p = _ll_params(0.2, 1e12, 100)
+    return Case("footing2d_prandtl", m, [mat], p, footing=(0.0, 1.0, c), gravity=False,
+                extra={"Nc": prandtl_nc(phi)})
+
+
+def footing3d_hetero(h: float = 2.0, z_len: float = 20.0, width: float = 4.0, q: float = 100.0) -> Case:
+    """3D heterogeneous slope (the four horizontal layers of the P4 configuration, psi = 0) under
+    its own weight, with a strip footing of the given width at the symmetry plane x = 0 of the crest
+    across the whole z extent, pressure q (kPa) scaled by the load factor t up to collapse (gravity
+    held constant).  P2 tetrahedra."""
+    m = M.slope_3d(h=h, z_len=z_len, layers=(15.0, 10.0, 5.0))
+    p = _ll_params(0.2, 1e12, 100)
+    return Case("footing3d_hetero", m, list(HETERO_LAYERS), p, footing=(0.0, width, q), gravity=True)
+
+
 def slope3d(h: float = 2.0, nz: int = 10, hetero: bool = False) -> Case:
     layers = (15.0, 10.0, 5.0) if hetero else None
     m = M.slope_3d(h=h, nz=nz, layers=layers)
@@ -136,6 +173,7 @@ class Solver:
     f: torch.Tensor
     free_mask: torch.Tensor
     linear: DeflatedFGMRES
+    f_const: torch.Tensor | None = None
 
 
 def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_refresh: int = 0,
@@ -148,6 +186,12 @@ def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_
     op = ConstitutiveOperator.from_mesh(case.mesh, case.materials, Davis_type=case.davis, fp32_shadow=use32)
     f_field = op.gravity_load(case.mesh)
     f = to_dev(f_field.T.reshape(-1).copy())
+    f_const = None
+    if case.footing is not None:
+        x0, x1, q = case.footing
+        F_foot, _ = strip_footing_load(case.mesh, (x0, x1), q)
+        f_const = f if case.gravity else None
+        f = to_dev(F_foot.T.reshape(-1).copy())
     free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
     if pc == "pmg" and case.mesh.elem_type == "P4":
         prec = PMultigridP4(case.mesh, op, use32=use32, coarse_refresh=coarse_refresh,
@@ -158,7 +202,7 @@ def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_
     else:
         prec = ChebyshevJacobi(use32=use32)
     lin = DeflatedFGMRES(prec, tolerance=case.params.linear_tol, max_iterations=case.params.linear_maxit)
-    return Solver(case, op, f, free_mask, lin)
+    return Solver(case, op, f, free_mask, lin, f_const)
 
 
 def run_ssr(s: Solver, *, predictor=None, store_step_u: bool = False, progress=None):
@@ -177,4 +221,4 @@ def run_ll(s: Solver, *, d_omega_ini_scale: float | None = None, d_t_min: float 
         d_t_min = getattr(p, "d_t_min", 1e-3)
     return ll_indirect_continuation(d_omega_ini_scale, d_t_min, p.step_max, p.omega_max_stop, p.it_newt_max,
                                     p.it_damp_max, p.tol, p.r_min, s.f, s.op, s.linear, free_mask=s.free_mask,
-                                    dim=s.case.mesh.dim, progress=progress)
+                                    dim=s.case.mesh.dim, progress=progress, f_const=s.f_const)

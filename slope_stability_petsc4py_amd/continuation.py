@@ -97,14 +97,16 @@ def init_phase_ssr(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_dam
 def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, f, builder, solver, *,
                               free_mask, dim, predictor=None, store_step_u=False, progress=None,
-                              predictor_weights=(1.0, 0.5, 0.25)) -> SSRResult:
+                              predictor_weights=(1.0, 0.5, 0.25), predictor_gate: str = "first_residual") -> SSRResult:
     """SSR indirect continuation; returns the lambda/omega/Umax histories (FoS = lambda_hist[-1]).
 
     ``predictor(ctx) -> U_guess or None`` is the learned-predictor slot: ctx holds the accepted
     states, histories and the omega target; it returns U_guess or (U_guess, lambda_guess).  The
     guess is blended with the secant guess at the weights ``predictor_weights`` (1 = the guess
     itself), the blend with the smallest first residual (at lambda_guess, default the current
-    lambda) is tried first and kept only if that residual is below the secant's (the gate)."""
+    lambda) is tried first.  ``predictor_gate`` "first_residual": it is kept only if that residual
+    is below the secant's; "newton_progress": Newton starts from it and the attempt is dropped
+    (the secant then used) when the first Newton step does not reduce the residual."""
     t0 = perf_counter()
     stats = NewtonStats()
     scr = _Scratch(f.device)
@@ -166,10 +168,11 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
                     best, r_l, gamma_best = cand, r_c, gamma
             learned = best
             gate_info = {"secant_first_residual": r_sec, "learned_first_residual": r_l, "learned_weight": gamma_best}
-            if r_l < r_sec:
+            if r_l < r_sec or predictor_gate == "newton_progress":
                 kind = "learned"
                 result = newton_ind_ssr(learned, omega_it, lam_guess, it_newt_max, it_damp_max, tol, r_min, f,
-                                        builder, solver, free_mask=free_mask, stats=stats)
+                                        builder, solver, free_mask=free_mask, stats=stats,
+                                        progress_gate=predictor_gate == "newton_progress")
                 if result[2] != 0:
                     solver.restore_deflation_basis(basis_snap)
                     result = None
@@ -247,20 +250,54 @@ class LLResult:
         return float(self.t_hist[-1])
 
 
+class LoadOffsetBuilder:
+    """A constitutive builder whose internal forces are F(U) - f_const: the limit-load continuation
+    then solves F(U) = f_const + t f (a constant load, e.g. gravity, under a growing one, e.g. a
+    footing pressure) with the reference's algorithm unchanged.  Every other call is forwarded."""
+
+    def __init__(self, builder, f_const: torch.Tensor):
+        self._b = builder
+        self._f0 = f_const
+
+    def __getattr__(self, name):
+        return getattr(self._b, name)
+
+    def build_F_vec(self, *a, **k):
+        return self._b.build_F_vec(*a, **k) - self._f0
+
+    def build_F_reduced_vec(self, *a, **k):
+        return self._b.build_F_reduced_vec(*a, **k) - self._f0
+
+    def build_F_all_vec(self, *a, **k):
+        return self._b.build_F_all_vec(*a, **k) - self._f0
+
+
 def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_stop, it_newt_max, it_damp_max, tol,
                              r_min, f, builder, solver, *, free_mask, dim, lambda_ell: float = 1.0,
-                             progress=None) -> LLResult:
+                             progress=None, f_const: torch.Tensor | None = None) -> LLResult:
     """Limit-load continuation in omega = f^T U (continuation/limit_load.py:66-374 with the driver's
     elastic start, cli/run_3D_hetero_SSR_capture.py:1224-1305): one elastic solve K_el U_el = f
     (strength reduced by lambda_ell), omega_el = f^T U_el, start from U = scale*U_el, t = 0, first
     omega step scale*omega_el; secant predictor in omega; d_omega halves on failure (5 in a row
     stop), doubles after an easy step (< 20 Newton iterations) whose t increment shrank by > 10%;
-    stop at d_t < d_t_min, omega_max_stop or step_max.  The limit load factor is t_hist[-1]."""
+    stop at d_t < d_t_min, omega_max_stop or step_max.  The limit load factor is t_hist[-1].
+
+    ``f_const`` (not in the reference, whose LL scales its one load): a constant load carried
+    throughout, F(U) = f_const + t f.  The branch then starts from the equilibrium U_0 under f_const
+    alone (plain Newton at t = 0), omega_0 = f^T U_0, and the elastic increment is added to U_0."""
     t0 = perf_counter()
     stats = NewtonStats()
     scr = _Scratch(f.device)
     lin0 = solver.iteration_collector.get_total_iterations()
     builder.reduction(float(lambda_ell))
+    U_0 = None
+    if f_const is not None:
+        U_0, flag0, _ = newton(torch.zeros_like(f), tol, it_newt_max, it_damp_max, r_min, f_const, builder, solver,
+                               free_mask=free_mask, stats=stats)
+        if flag0 != 0:
+            raise RuntimeError("limit load: no equilibrium under the constant load alone")
+        solver.expand_deflation_basis(U_0)
+        builder = LoadOffsetBuilder(builder, f_const)
     K_el = builder.elastic_matrix()
     solver.setup_preconditioner(K_el)
     solver.A_orthogonalize(K_el)
@@ -269,10 +306,13 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
     omega_el = dot(f, U_el, scr)
     d_omega = omega_el * float(d_omega_ini_scale)
     U = U_el * float(d_omega_ini_scale)
-    U_old = U.clone()
     omega = omega_old = 0.0
+    if U_0 is not None:
+        U = U + U_0
+        omega = omega_old = dot(f, U_0, scr)
+    U_old = U.clone()
     t = 0.0
-    t_hist, omega_hist, umax_hist = [0.0], [0.0], [_umax(U, dim)]
+    t_hist, omega_hist, umax_hist = [0.0], [omega], [_umax(U if U_0 is None else U_0, dim)]
     step = 1
     n_omega = 0
     stop_reason = "completed"

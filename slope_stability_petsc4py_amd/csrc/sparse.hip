@@ -150,16 +150,27 @@ __global__ void k_cheb_update(int64_t n, const T *__restrict__ dinv, const T *__
     x[i] += di;
 }
 
-// Extract the inverse diagonal of a CSR matrix (rows with a zero diagonal get 0).
+// Extract the inverse diagonal of a CSR matrix (rows with a zero diagonal get 0): kDiagLanes lanes
+// per row search the row's columns together (coalesced), the one that holds the diagonal
+// contributes its value to a shuffle sum (the others contribute 0).
+constexpr int kDiagLanes = 16;
+
 template <typename T>
-__global__ void k_inv_diag(int64_t n_rows, int64_t col0, const int32_t *__restrict__ indptr,
-                           const int32_t *__restrict__ indices, const T *__restrict__ val, T *__restrict__ dinv) {
-    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n_rows) return;
+__global__ void __launch_bounds__(256) k_inv_diag(int64_t n_rows, int64_t col0, const int32_t *__restrict__ indptr,
+                                                  const int32_t *__restrict__ indices, const T *__restrict__ val,
+                                                  T *__restrict__ dinv) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t r = tid / kDiagLanes;
+    const int lane = (int)(tid % kDiagLanes);
     T d = 0;
-    for (int32_t k = indptr[r]; k < indptr[r + 1]; ++k)
-        if ((int64_t)indices[k] == r + col0) d = val[k];
-    dinv[r] = d != (T)0 ? (T)1 / d : (T)0;
+    if (r < n_rows) {
+        const int32_t k1 = indptr[r + 1];
+        for (int32_t k = indptr[r] + lane; k < k1; k += kDiagLanes)
+            if ((int64_t)indices[k] == r + col0) d = val[k];
+    }
+#pragma unroll
+    for (int off = kDiagLanes / 2; off > 0; off >>= 1) d += __shfl_down(d, off, kDiagLanes);
+    if (r < n_rows && lane == 0) dinv[r] = d != (T)0 ? (T)1 / d : (T)0;
 }
 
 }  // namespace
@@ -268,7 +279,7 @@ int ssx_cheb_update_f64(int64_t n, const double *dinv, const double *r, double c
 int ssx_inv_diag_f32(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const float *val,
                      float *dinv, void *stream) {
     if (n_rows <= 0) return SSX_OK;
-    hipLaunchKernelGGL(k_inv_diag<float>, dim3(ssx::grid_for(n_rows, 256)), dim3(256), 0, ssx::as_stream(stream),
+    hipLaunchKernelGGL(k_inv_diag<float>, dim3(ssx::grid_for(n_rows * kDiagLanes, 256)), dim3(256), 0, ssx::as_stream(stream),
                        n_rows, col0, indptr, indices, val, dinv);
     SSX_CHECK_LAUNCH("ssx_inv_diag_f32");
     return SSX_OK;
@@ -277,7 +288,7 @@ int ssx_inv_diag_f32(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const int32_t *indices, const double *val,
                      double *dinv, void *stream) {
     if (n_rows <= 0) return SSX_OK;
-    hipLaunchKernelGGL(k_inv_diag<double>, dim3(ssx::grid_for(n_rows, 256)), dim3(256), 0, ssx::as_stream(stream),
+    hipLaunchKernelGGL(k_inv_diag<double>, dim3(ssx::grid_for(n_rows * kDiagLanes, 256)), dim3(256), 0, ssx::as_stream(stream),
                        n_rows, col0, indptr, indices, val, dinv);
     SSX_CHECK_LAUNCH("ssx_inv_diag_f64");
     return SSX_OK;

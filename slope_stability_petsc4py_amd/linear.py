@@ -503,10 +503,18 @@ class PMultigrid:
         if L is None:
             L, info = torch.linalg.cholesky_ex(Ac)
             if int(info.item()) != 0:
-                # indefinite coarse operator (strong softening): shifted factorisation
-                shift = 1e-8 * float(Ac.diagonal().abs().max().item())
-                Ac.diagonal().add_(shift)
-                L = torch.linalg.cholesky(Ac)
+                # indefinite coarse operator (strong softening): shifted factorisation, the shift
+                # raised by 100x until the factorisation succeeds (the coarse level only has to be
+                # a preconditioner for the outer FGMRES)
+                dmax = float(Ac.diagonal().abs().max().item()) or 1.0
+                for rel_shift in (1e-8, 1e-6, 1e-4, 1e-2, 1.0):
+                    Ac.diagonal().add_(rel_shift * dmax)
+                    L, info = torch.linalg.cholesky_ex(Ac)
+                    tm["coarse_shifts"] = tm.get("coarse_shifts", 0) + 1
+                    if int(info.item()) == 0:
+                        break
+                else:
+                    raise RuntimeError("p-multigrid: coarse operator not positive definite after shifting")
         del Ac
         torch.cuda.synchronize()
         t2 = perf_counter()

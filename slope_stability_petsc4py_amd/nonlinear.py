@@ -152,11 +152,13 @@ def newton(U_ini, tol, it_newt_max, it_damp_max, r_min, f, builder, solver, *, f
 
 
 def newton_ind_ssr(U_ini, omega, lambda_it, it_newt_max, it_damp_max, tol, r_min, f, builder, solver, *,
-                   free_mask, stats: NewtonStats | None = None, first_residual_gate: float | None = None):
+                   free_mask, stats: NewtonStats | None = None, first_residual_gate: float | None = None,
+                   progress_gate: bool = False):
     """Nested Newton for F_lambda(U) = f with f^T U = omega.  Returns (U, lambda, flag, it, info).
 
     ``first_residual_gate``: if given, the attempt is abandoned (flag 2) when the first relative
-    residual exceeds it -- the acceptance test of a learned initial guess."""
+    residual exceeds it.  ``progress_gate``: the attempt is abandoned (flag 2) when the first Newton
+    step does not reduce the residual.  Both are acceptance tests of a learned initial guess."""
     scr = _Scratch(U_ini.device)
     U = U_ini.clone()
     norm_f = _norm(f, scr) or 1.0
@@ -184,6 +186,9 @@ def newton_ind_ssr(U_ini, omega, lambda_it, it_newt_max, it_damp_max, tol, r_min
             if first_residual_gate is not None and not (rel < first_residual_gate):
                 flag = 2
                 break
+        elif it == 2 and progress_gate and not (rel < info["first_rel_residual"]):
+            flag = 2
+            break
         if stats is not None:
             stats.residual_evals += 1
         if compute_diffs and rel < tol and it > 1:

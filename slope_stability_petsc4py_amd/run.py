@@ -20,7 +20,8 @@ from . import cases
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--case", default="homo2d", choices=["homo2d", "hetero3d_L1", "slope3d_homo", "slope3d_hetero",
-                                                         "homo2d_ll", "homo3d_ll_L1", "hetero3d_ll_L1"])
+                                                         "homo2d_ll", "homo3d_ll_L1", "hetero3d_ll_L1",
+                                                         "footing2d_prandtl", "footing3d_hetero"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--pc", default="pmg", choices=["pmg", "cheb"])
     ap.add_argument("--analysis", default="ssr", choices=["ssr", "ll"])
@@ -40,7 +41,7 @@ def main(argv=None):
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
-    if a.case.endswith("_ll") or a.case.endswith("_ll_L1"):
+    if a.case.endswith("_ll") or a.case.endswith("_ll_L1") or a.case.startswith("footing"):
         a.analysis = "ll"
     if a.case == "homo2d":
         case = cases.homo2d(h=a.h or 1.0)
@@ -52,6 +53,10 @@ def main(argv=None):
         case = cases.hetero3d_ll_L1(a.elem_type)
     elif a.case == "hetero3d_L1":
         case = cases.hetero3d_L1(a.elem_type)
+    elif a.case == "footing2d_prandtl":
+        case = cases.footing2d_prandtl(h=a.h or 0.25)
+    elif a.case == "footing3d_hetero":
+        case = cases.footing3d_hetero(h=a.h or 2.0)
     else:
         case = cases.slope3d(h=a.h or 2.0, nz=a.nz or 10, hetero=a.case.endswith("hetero"))
     if a.step_max:
@@ -87,7 +92,7 @@ def main(argv=None):
                "stop_reason": res.stop_reason, "newton_iterations": res.newton_iterations,
                "linear_iterations": res.linear_iterations, "n_dof": int(s.op.pattern.n_rows),
                "n_elem": int(case.mesh.n_elem), "linear_solver": s.linear.iteration_collector.summary(),
-               "pc_timings": getattr(s.linear.pc, "timings", None)}
+               "pc_timings": getattr(s.linear.pc, "timings", None), "extra": case.extra}
     print(json.dumps(out), flush=True)
 
 

@@ -227,3 +227,36 @@ def test_oracle_a_orthogonalize_pins():
     D = np.diag(np.r_[-1.0, np.ones(n - 1)])
     W2, S2 = oracle.a_orthogonalize(np.eye(n)[:1] * 3.0, D)
     assert S2.tolist() == [-1.0] and abs(W2[0, 0] - 1.0) < 1e-15
+
+
+def test_load_offset_builder_subtracts_constant_load():
+    """Limit load with a constant load: every internal-force call returns F(U) - f_const and every
+    other call is forwarded unchanged."""
+    import torch
+    from slope_stability_petsc4py_amd.continuation import LoadOffsetBuilder
+
+    class Fake:
+        def __init__(self):
+            self.calls = []
+
+        def build_F_vec(self):
+            return torch.tensor([1.0, 2.0])
+
+        def build_F_reduced_vec(self, U):
+            return U * 2.0
+
+        def build_F_all_vec(self, lam, U):
+            return U * lam
+
+        def reduction(self, lam):
+            self.calls.append(lam)
+            return "ok"
+
+    fb = Fake()
+    f0 = torch.tensor([0.5, -1.0])
+    b = LoadOffsetBuilder(fb, f0)
+    U = torch.tensor([3.0, 4.0])
+    assert torch.equal(b.build_F_vec(), torch.tensor([0.5, 3.0]))
+    assert torch.equal(b.build_F_reduced_vec(U), torch.tensor([5.5, 9.0]))
+    assert torch.equal(b.build_F_all_vec(2.0, U), torch.tensor([5.5, 9.0]))
+    assert b.reduction(1.5) == "ok" and fb.calls == [1.5]

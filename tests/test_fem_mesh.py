@@ -181,3 +181,37 @@ def test_logical_ids_follow_physical_names():
     np.testing.assert_array_equal(mesh.logical_ids(np.array([2, 7]), {}, 2, "boundary"), [2, 7])
     with pytest.raises(ValueError):
         mesh.logical_ids(np.array([9]), names, 2, "boundary")
+
+
+def test_strip_footing_load_totals_and_p2_weights():
+    """Uniform pressure on a top strip: total force = q x loaded width (x depth in 3D); on P2 edges
+    the end/midside weights are L/6 and 2L/3, on P2 faces the corners carry nothing (exact integrals
+    of the quadratic shape functions)."""
+    from slope_stability_petsc4py_amd import loads
+    m = mesh.slope_2d_p2(h=0.5, x1=12.0, x2=1.0, x3=3.0, y1=8.0, y2=1.0)
+    F, w = loads.strip_footing_load(m, (0.0, 2.0), q=10.0)
+    assert np.isclose(F.sum(), -10.0 * w) and w > 1.5 and F[0].sum() == 0.0
+    yt = m.coord[1].max()
+    top = np.flatnonzero((np.abs(m.coord[1] - yt) < 1e-9) & (m.coord[0] <= w + 1e-9))
+    top = top[np.argsort(m.coord[0, top])]
+    fy = F[1, top]
+    L = m.coord[0, top[2]] - m.coord[0, top[0]]
+    assert np.isclose(fy[0], -10.0 * L / 6.0) and np.isclose(fy[1], -10.0 * 2.0 * L / 3.0)
+    assert np.isclose(fy[2], -10.0 * L / 3.0)           # shared vertex: two edges
+    m3 = mesh.slope_3d(h=1.0, x1=12.0, x2=1.0, x3=3.0, y1=8.0, y2=1.0, z_len=3.0, nz=3)
+    F3, w3 = loads.strip_footing_load(m3, (0.0, 2.0), q=10.0)
+    assert np.isclose(F3.sum(), -10.0 * w3 * 3.0)
+    corners = np.unique(m3.elem[:4].ravel())
+    assert np.all(F3[:, corners] == 0.0)
+    m1 = mesh.slope_3d(h=1.0, x1=12.0, x2=1.0, x3=3.0, y1=8.0, y2=1.0, z_len=3.0, nz=3, elem_type="P1")
+    F1, w1 = loads.strip_footing_load(m1, (0.0, 2.0), q=10.0)
+    assert np.isclose(F1.sum(), -10.0 * w1 * 3.0)
+
+
+def test_prandtl_bearing_capacity_factor():
+    """N_c closed form against its textbook values: 2 + pi at phi = 0, 30.14 at 30 deg, 14.83 at 20."""
+    from slope_stability_petsc4py_amd.cases import prandtl_nc
+    assert np.isclose(prandtl_nc(0.0), 2.0 + np.pi)
+    assert abs(prandtl_nc(30.0) - 30.14) < 0.01
+    assert abs(prandtl_nc(20.0) - 14.83) < 0.01
+    assert abs(prandtl_nc(1e-6) - (2.0 + np.pi)) < 1e-4

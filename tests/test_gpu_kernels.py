@@ -479,3 +479,31 @@ def test_sym_tiles_gemv_vs_numpy(gpu, n, f64):
         _lib.call("ssx_sym_tiles_pack", n, ptr(A32), 0, ptr(tiles), 0, 0)
         _lib.call("ssx_sym_tiles_gemv_f32", n, ptr(tiles), ptr(xd), ptr(part), ptr(y), 0)
         assert _rel(y.cpu().numpy().astype(np.float64), exp) < 2e-6
+
+
+@pytest.mark.parametrize("f64", [False, True])
+def test_inv_diag_vs_scipy(gpu, asm3d, f64):
+    """Inverse diagonal of the CSR operator (Jacobi/Chebyshev smoother) == 1 / diag, 0 on a zero
+    diagonal."""
+    from slope_stability_petsc4py_amd.device import ptr
+    m, geom, pat = asm3d
+    dp = DevicePattern(pat, geom)
+    rng = np.random.default_rng(31)
+    val = rng.uniform(0.5, 2.0, pat.nnz)
+    diag_pos = [pat.indptr[i] + int(np.flatnonzero(pat.indices[pat.indptr[i]:pat.indptr[i + 1]] == i)[0])
+                for i in (0, 7, pat.n_rows - 1)]
+    val[diag_pos[1]] = 0.0
+    K = dp.new_csr()
+    K.values.copy_(torch.from_numpy(val))
+    K.values32.copy_(torch.from_numpy(val.astype(np.float32)))
+    dt = torch.float64 if f64 else torch.float32
+    dinv = torch.full((pat.n_rows,), np.nan, dtype=dt, device=gpu)
+    _lib.call("ssx_inv_diag_f64" if f64 else "ssx_inv_diag_f32", pat.n_rows, 0, ptr(K.indptr), ptr(K.indices),
+              ptr(K.values if f64 else K.values32), ptr(dinv), 0)
+    import scipy.sparse as sp
+    A = sp.csr_matrix((val, pat.indices, pat.indptr), shape=(pat.n_rows, pat.n_rows))
+    d = A.diagonal()
+    exp = np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    got = dinv.cpu().numpy().astype(np.float64)
+    assert got[7] == 0.0
+    np.testing.assert_allclose(got, exp, rtol=1e-15 if f64 else 1e-6, atol=0)

@@ -70,3 +70,42 @@ def test_adam_matches_oracle(gpu):
     net.adam(1e-2)
     w1, _, _ = O.adam(w0, g.astype(np.float64), np.zeros_like(w0), np.zeros_like(w0), 1, lr=1e-2)
     np.testing.assert_allclose(net.W[0].cpu().numpy(), w1, rtol=1e-5, atol=1e-6)
+
+
+def test_learned_predictor_gain_on_hetero_ssr(gpu):
+    """What the trained network gains over the method without it: the network is trained (device
+    forward/backward/Adam) on the continuation histories of three perturbed-material runs of the
+    3D heterogeneous L1 slope, then fills the predictor slot of the SSR continuation on the
+    reference materials (FP32 form), against the same continuation with the secant predictor.
+    The Newton-iteration and wall-time ratios are printed (the gain); the factor of safety must
+    not move by more than 1e-3, and the first-residual gate must keep the learned form within 10%
+    of the plain form's Newton iterations."""
+    from slope_stability_petsc4py_amd import cases
+    from slope_stability_petsc4py_amd.learn import LearnedPredictor, node_info, samples_from_history, train
+    from slope_stability_petsc4py_amd.learn_experiment import hetero_variant
+    print("case: learned predictor on hetero3d_L1", flush=True)
+    samples = []
+    for seed in range(3):
+        case = hetero_variant(seed)
+        res = cases.run_ssr(cases.make_solver(case, precision="fp32"), store_step_u=True)
+        samples += samples_from_history(node_info(case.mesh, case.materials), res.step_U, res.omega_hist,
+                                        res.lambda_hist)
+        del res
+    assert len(samples) >= 10
+    net = NodeMLP(seed=0)
+    losses = train(net, samples, epochs=60, lr=2e-3)
+    assert losses[-1] < losses[0]
+    ref = hetero_variant(-1)
+    plain = cases.run_ssr(cases.make_solver(ref, precision="fp32"))
+    pred = LearnedPredictor(net, node_info(ref.mesh, ref.materials), smoothing=2)
+    lrn = cases.run_ssr(cases.make_solver(ref, precision="fp32"), predictor=pred)
+    kinds = [r["predictor"] for r in lrn.step_info]
+    print("gain: newton", plain.newton_iterations, "->", lrn.newton_iterations,
+          f"({plain.newton_iterations / max(lrn.newton_iterations, 1):.3f}x),",
+          "wall", round(plain.wall_time, 3), "->", round(lrn.wall_time, 3),
+          f"({plain.wall_time / lrn.wall_time:.3f}x);", "steps used / gated:", kinds.count("learned"),
+          "/", kinds.count("learned_gated"), "; fos", plain.fos, lrn.fos, flush=True)
+    assert pred.calls > 0
+    assert abs(lrn.fos - plain.fos) < 1e-3
+    assert abs(lrn.fos - 1.6661099309952119) < 1e-3
+    assert lrn.newton_iterations <= 1.1 * plain.newton_iterations + 2
