@@ -127,6 +127,15 @@ int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const
                         const int32_t *indices, const double *val, int lanes, double tol, double *W, double *AW,
                         double *sgn, double *work, void *stream);
 
+/* One fused Chebyshev(Jacobi) smoothing step on CSR A: r = b - A x_in (x_in NULL: zero), d = a d +
+ * c dinv r (first != 0: d = c dinv r), x_out = x_in + d; x_out must differ from x_in.  Replaces the
+ * residual, smoother and update calls of one step of linear/solver.py:155-162,417-436. */
+int ssx_cheb_step_f32(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const float *val,
+                      const float *b, const float *x_in, float *x_out, float *d, const float *dinv, float c, float a,
+                      int first, int lanes, void *stream);
+int ssx_cheb_step_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                      const double *b, const double *x_in, double *x_out, double *d, const double *dinv, double c,
+                      double a, int first, int lanes, void *stream);
 /* Chebyshev/Jacobi smoother step: d = a*d + c*dinv.*r; x += d (replaces the Richardson/SOR and
  * Chebyshev/Jacobi smoother applies of linear/solver.py:155-162,417-436). */
 int ssx_cheb_update_f32(int64_t n, const float *dinv, const float *r, float c, float a, float *x, float *d,

@@ -38,6 +38,8 @@ _SIGS: dict[str, list] = {
     "ssx_dot": [I64, P, P, P, P, P],
     "ssx_gemv_cols": [I64, I32, P, I64, P, F64, P, P],
     "ssx_spmm_f64": [I64, P, P, P, I32, P, I64, P, I64, I32, P],
+    "ssx_cheb_step_f32": [I64, P, P, P, P, P, P, P, P, F32, F32, I32, I32, P],
+    "ssx_cheb_step_f64": [I64, P, P, P, P, P, P, P, P, F64, F64, I32, I32, P],
     "ssx_sym_tiles_sizes": [I64, P, P],
     "ssx_sym_tiles_pack": [I64, P, I32, P, I32, P],
     "ssx_sym_tiles_gemv_f32": [I64, P, P, P, P, P],

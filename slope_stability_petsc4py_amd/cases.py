@@ -143,7 +143,8 @@ def footing2d_prandtl(h: float = 0.25, phi: float = 10.0, c: float = 10.0) -> Ca
     domain holds the Prandtl mechanism (a few metres wide and deep) well inside."""
     m = M.slope_2d_p2(h=h, x1=12.0, x2=1.0, x3=3.0, y1=8.0, y2=1.0)
     mat = MaterialSpec(c, phi, phi, 40000.0, 0.30, 0.0, 0.0)
-    p = _ll_params(0.2, 1e12, 100)
+    p = _ll_params(1.0, 1e12, 100)
+    p.step_max = 400
     return Case("footing2d_prandtl", m, [mat], p, footing=(0.0, 1.0, c), gravity=False,
                 extra={"Nc": prandtl_nc(phi)})
 
@@ -154,7 +155,8 @@ def footing3d_hetero(h: float = 2.0, z_len: float = 20.0, width: float = 4.0, q:
     across the whole z extent, pressure q (kPa) scaled by the load factor t up to collapse (gravity
     held constant).  P2 tetrahedra."""
     m = M.slope_3d(h=h, z_len=z_len, layers=(15.0, 10.0, 5.0))
-    p = _ll_params(0.2, 1e12, 100)
+    p = _ll_params(0.5, 1e12, 100)
+    p.step_max = 400
     return Case("footing3d_hetero", m, list(HETERO_LAYERS), p, footing=(0.0, width, q), gravity=True)
 
 

@@ -150,6 +150,53 @@ __global__ void k_cheb_update(int64_t n, const T *__restrict__ dinv, const T *__
     x[i] += di;
 }
 
+// One fused Chebyshev(Jacobi) step: per row i (G lanes, as in the SpMV)
+//   r_i = b_i - (A x_in)_i  (x_in == nullptr: the zero vector, r = b)
+//   d_i = a d_i + c dinv_i r_i  (first: d_i = c dinv_i r_i, the old d not read)
+//   x_out_i = x_in_i + d_i
+// x_in and x_out are different buffers (every row reads all of x_in while rows write x_out).
+template <typename T, int G>
+__global__ void __launch_bounds__(256) k_cheb_step(int64_t n_rows, const int32_t *__restrict__ indptr,
+                                                   const int32_t *__restrict__ indices, const T *__restrict__ val,
+                                                   const T *__restrict__ b, const T *__restrict__ x_in,
+                                                   T *__restrict__ x_out, T *__restrict__ d,
+                                                   const T *__restrict__ dinv, T c, T a, int first) {
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t row = tid / G;
+    const int lane = (int)(tid % G);
+    T acc = 0;
+    if (row < n_rows && x_in != nullptr) {
+        const int32_t k1 = indptr[row + 1];
+        for (int32_t k = indptr[row] + lane; k < k1; k += G) acc += val[k] * x_in[indices[k]];
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, G);
+    if (row < n_rows && lane == 0) {
+        const T r = b[row] - acc;
+        const T di = first ? c * dinv[row] * r : a * d[row] + c * dinv[row] * r;
+        d[row] = di;
+        x_out[row] = (x_in != nullptr ? x_in[row] : (T)0) + di;
+    }
+}
+
+template <typename T>
+int launch_cheb_step(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const T *val, const T *b,
+                     const T *x_in, T *x_out, T *d, const T *dinv, T c, T a, int first, int lanes, hipStream_t st) {
+    const dim3 grid(ssx::grid_for(n_rows * lanes, 256)), block(256);
+    switch (lanes) {
+        case 1: hipLaunchKernelGGL((k_cheb_step<T, 1>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        case 2: hipLaunchKernelGGL((k_cheb_step<T, 2>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        case 4: hipLaunchKernelGGL((k_cheb_step<T, 4>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        case 8: hipLaunchKernelGGL((k_cheb_step<T, 8>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        case 16: hipLaunchKernelGGL((k_cheb_step<T, 16>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        case 32: hipLaunchKernelGGL((k_cheb_step<T, 32>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        case 64: hipLaunchKernelGGL((k_cheb_step<T, 64>), grid, block, 0, st, n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first); break;
+        default: ssx::set_error("ssx_cheb_step: lanes per row must be a power of two <= 64"); return SSX_EINVAL;
+    }
+    SSX_CHECK_LAUNCH("ssx_cheb_step");
+    return SSX_OK;
+}
+
 // Extract the inverse diagonal of a CSR matrix (rows with a zero diagonal get 0): kDiagLanes lanes
 // per row search the row's columns together (coalesced), the one that holds the diagonal
 // contributes its value to a shuffle sum (the others contribute 0).
@@ -256,6 +303,28 @@ int ssx_gemv_cols(int64_t n, int k, const double *V, int64_t ld, const double *h
                        sign, x);
     SSX_CHECK_LAUNCH("ssx_gemv_cols");
     return SSX_OK;
+}
+
+int ssx_cheb_step_f32(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const float *val,
+                      const float *b, const float *x_in, float *x_out, float *d, const float *dinv, float c, float a,
+                      int first, int lanes, void *stream) {
+    SSX_REQUIRE(n_rows >= 0, "ssx_cheb_step_f32: n_rows < 0");
+    if (n_rows == 0) return SSX_OK;
+    SSX_REQUIRE(indptr && indices && val && b && x_out && d && dinv && x_out != x_in,
+                "ssx_cheb_step_f32: null pointer or x_out == x_in");
+    return launch_cheb_step<float>(n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first, lanes,
+                                   ssx::as_stream(stream));
+}
+
+int ssx_cheb_step_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                      const double *b, const double *x_in, double *x_out, double *d, const double *dinv, double c,
+                      double a, int first, int lanes, void *stream) {
+    SSX_REQUIRE(n_rows >= 0, "ssx_cheb_step_f64: n_rows < 0");
+    if (n_rows == 0) return SSX_OK;
+    SSX_REQUIRE(indptr && indices && val && b && x_out && d && dinv && x_out != x_in,
+                "ssx_cheb_step_f64: null pointer or x_out == x_in");
+    return launch_cheb_step<double>(n_rows, indptr, indices, val, b, x_in, x_out, d, dinv, c, a, first, lanes,
+                                    ssx::as_stream(stream));
 }
 
 int ssx_cheb_update_f32(int64_t n, const float *dinv, const float *r, float c, float a, float *x, float *d,

@@ -67,6 +67,25 @@ def axpby(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> None:
         _lib.call("ssx_axpby_f32", y.numel(), float(a), ptr(x), float(b), ptr(y), _st())
 
 
+def lower_triangular_inverse(L: torch.Tensor, base: int = 512) -> torch.Tensor:
+    """Inverse of a lower-triangular matrix by recursive 2x2 blocks:
+    [[A, 0], [C, D]]^-1 = [[A^-1, 0], [-D^-1 C A^-1, D^-1]]; blocks of at most `base` rows by a
+    triangular solve against the identity."""
+    n = L.shape[0]
+    if n <= base:
+        return torch.linalg.solve_triangular(L, torch.eye(n, dtype=L.dtype, device=L.device), upper=False)
+    k = ((n // 2 + 63) // 64) * 64          # split on a multiple of 64 rows when that is a split
+    if k >= n:
+        k = n // 2
+    Ai = lower_triangular_inverse(L[:k, :k], base)
+    Di = lower_triangular_inverse(L[k:, k:], base)
+    out = torch.zeros_like(L)
+    out[:k, :k] = Ai
+    out[k:, k:] = Di
+    out[k:, :k] = -(Di @ (L[k:, :k] @ Ai))
+    return out
+
+
 def sym_tile_sizes(n: int) -> tuple[int, int]:
     """Element counts of the symmetric tile store of an n x n matrix and of its apply's partials."""
     import ctypes
@@ -137,7 +156,9 @@ class _Level:
         self.r = torch.empty(n, dtype=self.dtype, device=dev)
         self.d = torch.empty(n, dtype=self.dtype, device=dev)
         self.x = torch.empty(n, dtype=self.dtype, device=dev)
+        self.xb = torch.empty(n, dtype=self.dtype, device=dev)    # second iterate buffer (fused steps)
         self.lmax = 2.0
+        self.fused = True
 
     def refresh(self, power_its: int = 12, seed: int = 0):
         A = self.A
@@ -159,6 +180,9 @@ class _Level:
 
     def cheb(self, b: torch.Tensor, x: torch.Tensor, degree: int, lo_frac: float, zero_guess: bool):
         """`degree` Chebyshev(Jacobi) steps on A x = b over [lo_frac*lmax, lmax]."""
+        if self.fused:
+            self._cheb_fused(b, x, degree, lo_frac, zero_guess)
+            return
         beta = self.lmax
         alpha = lo_frac * self.lmax
         theta = 0.5 * (beta + alpha)
@@ -181,6 +205,38 @@ class _Level:
             _lib.call(upd, self.n, ptr(self.dinv), ptr(self.r), 2.0 * rho_new / delta, rho_new * rho, ptr(x),
                       ptr(self.d), _st())
             rho = rho_new
+
+    def _cheb_fused(self, b: torch.Tensor, x: torch.Tensor, degree: int, lo_frac: float, zero_guess: bool):
+        """The same steps as `cheb`, one launch each (`ssx_cheb_step_*`: residual, smoother and update
+        fused), the iterate alternating between x and a second buffer so that the last step writes x."""
+        beta = self.lmax
+        alpha = lo_frac * self.lmax
+        theta = 0.5 * (beta + alpha)
+        delta = 0.5 * (beta - alpha)
+        sigma = theta / delta
+        rho = 1.0 / sigma
+        A = self.A
+        fn = "ssx_cheb_step_f32" if self.use32 else "ssx_cheb_step_f64"
+        vals = A.values32 if self.use32 else A.values
+        other = {id(x): self.xb, id(self.xb): x}
+        # destination of step k: x when degree - 1 - k is even
+        dst = x if (degree - 1) % 2 == 0 else self.xb
+        src = None
+        if not zero_guess:
+            if dst is x:                 # the first step must not read the buffer it writes
+                self.xb.copy_(x)
+                src = self.xb
+            else:
+                src = x
+        c, a = 1.0 / theta, 0.0
+        for k in range(degree):
+            if k:
+                rho_new = 1.0 / (2.0 * sigma - rho)
+                c, a = 2.0 * rho_new / delta, rho_new * rho
+                rho = rho_new
+            _lib.call(fn, A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(vals), ptr(b), ptr(src), ptr(dst),
+                      ptr(self.d), ptr(self.dinv), c, a, 1 if k == 0 else 0, A.lanes, _st())
+            src, dst = dst, other[id(dst)]
 
 
 class ChebyshevJacobi:
@@ -414,6 +470,7 @@ class PMultigrid:
         self.coarse_refresh = max(0, int(coarse_refresh))
         self.refresh_iterations = int(refresh_iterations)
         self._stale = False
+        self._fresh_its = None    # iterations of the first solve after the latest coarse rebuild
         # precision of the coarse factorisation: FP64, or FP32 (falls back to FP64 when the FP32
         # Cholesky meets a non-positive pivot); default follows the level precision
         self.coarse_precision = coarse_precision or ("fp32" if use32 else "fp64")
@@ -470,13 +527,20 @@ class PMultigrid:
         self.timings["level_refresh"] += perf_counter() - t0
         due = self.coarse_refresh > 0 and self._setups % self.coarse_refresh == 0
         if self.Ainv is None or due or self._stale:
+            self._fresh_its = None
             self._coarse(A, dev)
             self._stale = False
         self._setups += 1
 
     def note_iterations(self, its: int) -> None:
-        """Called by the Krylov solver after each solve with its iteration count."""
-        if self.coarse_refresh == 0 and its > self.refresh_iterations:
+        """Called by the Krylov solver after each solve with its iteration count.  The coarse
+        operator is marked stale when a solve needs more than `refresh_iterations` iterations and
+        more than 1.5x what the first solve after the last rebuild needed (when even a fresh coarse
+        operator leaves many iterations, the fine level is what limits them, and a rebuild would
+        not help)."""
+        if self._fresh_its is None:
+            self._fresh_its = its
+        if self.coarse_refresh == 0 and its > max(self.refresh_iterations, 1.5 * self._fresh_its):
             self._stale = True
 
     def _coarse(self, A: DeviceCSR, dev) -> None:
@@ -519,8 +583,14 @@ class PMultigrid:
         torch.cuda.synchronize()
         t2 = perf_counter()
         tm["factor"] += t2 - t1
-        Ainv = torch.cholesky_inverse(L).contiguous()
+        # A^-1 = L^-T L^-1 with L^-1 by recursive 2x2 block inversion (GEMMs; small diagonal blocks
+        # by triangular solves): the library's Cholesky-solve and large triangular-solve routes fail
+        # inside the solver/BLAS libraries from a few thousand unknowns on (internal error, workspace
+        # allocation)
+        Linv = lower_triangular_inverse(L)
         del L
+        Ainv = (Linv.mT @ Linv).contiguous()
+        del Linv
         # the inverse is kept as the upper triangle of 64 x 64 tiles in the level precision
         dt = torch.float32 if self.use32 else torch.float64
         te, pe = sym_tile_sizes(self.n_c)

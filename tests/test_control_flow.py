@@ -260,3 +260,17 @@ def test_load_offset_builder_subtracts_constant_load():
     assert torch.equal(b.build_F_reduced_vec(U), torch.tensor([5.5, 9.0]))
     assert torch.equal(b.build_F_all_vec(2.0, U), torch.tensor([5.5, 9.0]))
     assert b.reduction(1.5) == "ok" and fb.calls == [1.5]
+
+
+def test_lower_triangular_inverse_recursive_blocks():
+    """Recursive 2x2-block inverse of a lower-triangular factor (the coarse-level inverse) == the
+    identity check L^-1 L = I, stays lower triangular, for sizes around the block boundaries."""
+    import torch
+    from slope_stability_petsc4py_amd.linear import lower_triangular_inverse
+    for n in (1, 63, 64, 65, 300):
+        rng = np.random.default_rng(n)
+        M = rng.normal(size=(n, n))
+        L = torch.linalg.cholesky(torch.from_numpy(M @ M.T + n * np.eye(n)))
+        Li = lower_triangular_inverse(L, base=32)
+        assert torch.allclose(Li @ L, torch.eye(n, dtype=torch.float64), atol=1e-12)
+        assert torch.all(torch.triu(Li, 1) == 0)

@@ -120,3 +120,51 @@ def test_ll_matches_reference_limit_load(gpu, case_name):
     assert abs(t[-1] - ref_t[-1]) <= 1e-3 * ref_t[-1]
     assert abs(om[1] / t[1] - ref_om[1] / ref_t[1]) <= 1e-3 * ref_om[1] / ref_t[1]
     assert np.all(np.diff(t[:-1]) > 0)
+
+
+@pytest.mark.parametrize("use32", [False, True])
+@pytest.mark.parametrize("degree", [2, 3])
+def test_fused_chebyshev_step_matches_unfused(gpu, elastic3d, use32, degree):
+    """One launch per Chebyshev(Jacobi) step (residual + smoother + update fused, iterate
+    ping-ponged) == the residual / update sequence it replaces, from a zero and a given guess."""
+    from slope_stability_petsc4py_amd.linear import _Level
+    print(f"case: fused chebyshev fp32={use32} degree={degree}", flush=True)
+    m, op, K, f = elastic3d
+    dt = torch.float32 if use32 else torch.float64
+    rng = np.random.default_rng(degree)
+    b = torch.from_numpy(rng.normal(size=K.n_rows)).to(gpu).to(dt)
+    x0 = torch.from_numpy(rng.normal(size=K.n_rows)).to(gpu).to(dt)
+    out = {}
+    for fused in (False, True):
+        lv = _Level(K, use32)
+        lv.refresh()
+        lv.fused = fused
+        xa = torch.empty_like(b)
+        lv.cheb(b, xa, degree, 0.1, zero_guess=True)
+        xb = x0.clone()
+        lv.cheb(b, xb, degree, 0.1, zero_guess=False)
+        out[fused] = (xa.double().cpu().numpy(), xb.double().cpu().numpy())
+    tol = 1e-5 if use32 else 1e-12
+    for i in range(2):
+        ref = out[False][i]
+        assert np.abs(out[True][i] - ref).max() <= tol * np.abs(ref).max()
+
+
+def test_footing_limit_load_converges_to_prandtl(gpu):
+    """Strip footing on a weightless associated Mohr-Coulomb layer (phi = 10 deg, c = 10 kPa,
+    pressure q = c scaled by t): the limit-load continuation's last t approaches Prandtl's N_c =
+    8.345 from above as the mesh is refined (measured 9.08 at h = 0.5 and 8.67 at h = 0.25), with
+    the error at least halving per refinement; this pins the surface load and the limit-load
+    continuation against a closed form."""
+    out = []
+    for h in (0.5, 0.25):
+        print(f"case: footing2d_prandtl h={h}", flush=True)
+        case = cases.footing2d_prandtl(h=h)
+        res = cases.run_ll(cases.make_solver(case, precision="fp64"))
+        out.append(res.limit_load)
+        print("t_limit", res.limit_load, "Nc", case.extra["Nc"], "steps", len(res.t_hist), "wall", res.wall_time,
+              flush=True)
+    nc = cases.prandtl_nc(10.0)
+    assert nc < out[1] < out[0]
+    assert out[1] - nc < 0.55 * (out[0] - nc)
+    assert out[1] < 1.05 * nc and out[0] < 1.10 * nc
