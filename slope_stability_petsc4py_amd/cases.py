@@ -152,10 +152,12 @@ def footing2d_prandtl(h: float = 0.25, phi: float = 10.0, c: float = 10.0) -> Ca
 def footing3d_hetero(h: float = 2.0, z_len: float = 20.0, width: float = 4.0, q: float = 100.0) -> Case:
     """3D heterogeneous slope (the four horizontal layers of the P4 configuration, psi = 0) under
     its own weight, with a strip footing of the given width at the symmetry plane x = 0 of the crest
-    across the whole z extent, pressure q (kPa) scaled by the load factor t up to collapse (gravity
-    held constant).  P2 tetrahedra."""
+    across the whole z extent, pressure q (kPa) scaled by the load factor t (gravity held constant),
+    followed to omega = f^T U = 3e4.  P2 tetrahedra."""
     m = M.slope_3d(h=h, z_len=z_len, layers=(15.0, 10.0, 5.0))
-    p = _ll_params(0.5, 1e12, 100)
+    # the load-displacement curve keeps rising slowly past the knee (6.86 at omega = 3e4, 6.94 at
+    # 2.4e5 on h = 1.5), so the branch is followed to a fixed omega, where forms are compared
+    p = _ll_params(0.5, 3e4, 100)
     p.step_max = 400
     return Case("footing3d_hetero", m, list(HETERO_LAYERS), p, footing=(0.0, width, q), gravity=True)
 
@@ -179,11 +181,13 @@ class Solver:
 
 
 def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_refresh: int = 0,
-                coarse_precision: str | None = None, refresh_iterations: int = 25) -> Solver:
+                coarse_precision: str | None = None, refresh_iterations: int = 20,
+                refresh_ratio: float = 1.5) -> Solver:
     """precision 'fp64': FP64 preconditioner; 'fp32': FP32 tangent shadow + FP32 preconditioner
     (the outer FGMRES, residuals and return mapping stay FP64).  `coarse_refresh`: the p-multigrid
     coarse operator is rebuilt every that many preconditioner setups (1 = every Newton iteration),
-    or, with 0 (default), when a solve needs more than `refresh_iterations` FGMRES iterations."""
+    or, with 0 (default), when a solve needs more than `refresh_iterations` FGMRES iterations and
+    more than `refresh_ratio` times the iterations of the first solve after the last rebuild."""
     use32 = precision == "fp32"
     op = ConstitutiveOperator.from_mesh(case.mesh, case.materials, Davis_type=case.davis, fp32_shadow=use32)
     f_field = op.gravity_load(case.mesh)
@@ -197,10 +201,11 @@ def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_
     free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
     if pc == "pmg" and case.mesh.elem_type == "P4":
         prec = PMultigridP4(case.mesh, op, use32=use32, coarse_refresh=coarse_refresh,
-                            refresh_iterations=refresh_iterations)
+                            refresh_iterations=refresh_iterations, refresh_ratio=refresh_ratio)
     elif pc == "pmg":
         prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32, coarse_refresh=coarse_refresh,
-                          coarse_precision=coarse_precision, refresh_iterations=refresh_iterations)
+                          coarse_precision=coarse_precision, refresh_iterations=refresh_iterations,
+                          refresh_ratio=refresh_ratio)
     else:
         prec = ChebyshevJacobi(use32=use32)
     lin = DeflatedFGMRES(prec, tolerance=case.params.linear_tol, max_iterations=case.params.linear_maxit)

@@ -389,7 +389,7 @@ class PMultigridP4:
     (Jacobi) smoothing on the P4 and P2 levels, FP32 or FP64."""
 
     def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 0, refresh_iterations: int = 25):
+                 coarse_refresh: int = 0, refresh_iterations: int = 20, refresh_ratio: float = 1.5):
         from .device import DevicePattern
         from .fem import geometry, quadrature_3d
         from .pattern import build_owned_pattern
@@ -407,7 +407,8 @@ class PMultigridP4:
         # per-point shear and lame into its element matrices
         self.A2 = self.pat2.new_csr(with32=True)
         self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac,
-                                 coarse_refresh=coarse_refresh, refresh_iterations=refresh_iterations)
+                                 coarse_refresh=coarse_refresh, refresh_iterations=refresh_iterations,
+                                 refresh_ratio=refresh_ratio)
         self.note_iterations = self.coarse.note_iterations
         self.timings = self.coarse.timings
         self.level = None
@@ -460,7 +461,8 @@ class PMultigrid:
     only costs iterations, never correctness)."""
 
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 25):
+                 coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 20,
+                 refresh_ratio: float = 1.5):
         self.use32 = use32
         self.degree = degree
         self.lo_frac = lo_frac
@@ -469,6 +471,7 @@ class PMultigrid:
         # correction has become too weak)
         self.coarse_refresh = max(0, int(coarse_refresh))
         self.refresh_iterations = int(refresh_iterations)
+        self.refresh_ratio = float(refresh_ratio)
         self._stale = False
         self._fresh_its = None    # iterations of the first solve after the latest coarse rebuild
         # precision of the coarse factorisation: FP64, or FP32 (falls back to FP64 when the FP32
@@ -535,12 +538,12 @@ class PMultigrid:
     def note_iterations(self, its: int) -> None:
         """Called by the Krylov solver after each solve with its iteration count.  The coarse
         operator is marked stale when a solve needs more than `refresh_iterations` iterations and
-        more than 1.5x what the first solve after the last rebuild needed (when even a fresh coarse
-        operator leaves many iterations, the fine level is what limits them, and a rebuild would
-        not help)."""
+        more than `refresh_ratio` times what the first solve after the last rebuild needed (when
+        even a fresh coarse operator leaves many iterations, the fine level is what limits them,
+        and a rebuild would not help)."""
         if self._fresh_its is None:
             self._fresh_its = its
-        if self.coarse_refresh == 0 and its > max(self.refresh_iterations, 1.5 * self._fresh_its):
+        if self.coarse_refresh == 0 and its > max(self.refresh_iterations, self.refresh_ratio * self._fresh_its):
             self._stale = True
 
     def _coarse(self, A: DeviceCSR, dev) -> None:
@@ -583,14 +586,19 @@ class PMultigrid:
         torch.cuda.synchronize()
         t2 = perf_counter()
         tm["factor"] += t2 - t1
-        # A^-1 = L^-T L^-1 with L^-1 by recursive 2x2 block inversion (GEMMs; small diagonal blocks
-        # by triangular solves): the library's Cholesky-solve and large triangular-solve routes fail
-        # inside the solver/BLAS libraries from a few thousand unknowns on (internal error, workspace
-        # allocation)
-        Linv = lower_triangular_inverse(L)
-        del L
-        Ainv = (Linv.mT @ Linv).contiguous()
-        del Linv
+        # A^-1: the library's Cholesky inverse while its right-hand side block (n x n) stays below
+        # 1.5 GB (above about 2 GB the solver library fails with an internal error); beyond that
+        # L^-T L^-1 with L^-1 by recursive 2x2 block inversion (GEMMs, diagonal blocks of <= 512 rows
+        # by triangular solves -- a large triangular solve fails to get its BLAS workspace)
+        n_l = L.shape[0]
+        if n_l * n_l * L.element_size() < 1.5 * 2 ** 30:
+            Ainv = torch.cholesky_inverse(L).contiguous()
+            del L
+        else:
+            Linv = lower_triangular_inverse(L)
+            del L
+            Ainv = (Linv.mT @ Linv).contiguous()
+            del Linv
         # the inverse is kept as the upper triangle of 64 x 64 tiles in the level precision
         dt = torch.float32 if self.use32 else torch.float64
         te, pe = sym_tile_sizes(self.n_c)

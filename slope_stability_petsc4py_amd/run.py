@@ -34,7 +34,7 @@ def main(argv=None):
     ap.add_argument("--coarse-refresh", type=int, default=0,
                     help="rebuild the multigrid coarse operator every N preconditioner setups (0: when a solve "
                          "needs more than --refresh-iterations iterations)")
-    ap.add_argument("--refresh-iterations", type=int, default=25,
+    ap.add_argument("--refresh-iterations", type=int, default=20,
                     help="FGMRES iterations of one solve above which the coarse operator is rebuilt")
     ap.add_argument("--coarse-precision", default=None, choices=["fp32", "fp64"],
                     help="coarse factorisation precision (default: the preconditioner precision)")
