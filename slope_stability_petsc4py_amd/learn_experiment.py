@@ -66,7 +66,7 @@ def hetero_main(a, log):
     t0 = time.perf_counter()
     samples, train_info = [], []
     for seed in range(0 if a.load else a.train):
-        case = hetero_variant(seed)
+        case = hetero_variant(seed, a.spread, a.dphi)
         s, res = run_case(case, precision="fp32", store=True)
         info = node_info(case.mesh, case.materials)
         samples += samples_from_history(info, res.step_U, res.omega_hist, res.lambda_hist)
@@ -134,6 +134,8 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=12)
     ap.add_argument("--test", type=int, default=4)
     ap.add_argument("--epochs", type=int, default=150)
+    ap.add_argument("--spread", type=float, default=0.2, help="hetero3d_L1 training: relative spread of c0")
+    ap.add_argument("--dphi", type=float, default=3.0, help="hetero3d_L1 training: spread of phi (degrees)")
     ap.add_argument("--h", type=float, default=2.0)
     ap.add_argument("--nz", type=int, default=6)
     a = ap.parse_args(argv)

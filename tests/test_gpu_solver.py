@@ -168,3 +168,24 @@ def test_footing_limit_load_converges_to_prandtl(gpu):
     assert nc < out[1] < out[0]
     assert out[1] - nc < 0.55 * (out[0] - nc)
     assert out[1] < 1.05 * nc and out[0] < 1.10 * nc
+
+
+def test_footing_under_gravity_fp32_matches_fp64(gpu):
+    """Limit load of a strip footing on the layered slope under its own weight (constant gravity
+    load, F(U) = f_g + t f): the FP32-preconditioner form follows the plain FP64 form's load path
+    -- t at equal omega within 1e-3 -- and the branch starts from the gravity equilibrium
+    (omega_0 = f^T U_g > 0, t = 0)."""
+    print("case: footing3d_hetero h=2", flush=True)
+    out = {}
+    for prec in ("fp64", "fp32"):
+        res = cases.run_ll(cases.make_solver(cases.footing3d_hetero(h=2.0), precision=prec))
+        out[prec] = (np.asarray(res.omega_hist), np.asarray(res.t_hist))
+        print(prec, "t", res.t_hist[-1], "omega", res.omega_hist[-1], "steps", len(res.t_hist), "newton",
+              res.newton_iterations, "wall", res.wall_time, flush=True)
+        assert res.t_hist[0] == 0.0 and res.omega_hist[0] > 0.0
+        assert np.all(np.diff(res.omega_hist) > 0)
+    w = min(out["fp64"][0][-1], out["fp32"][0][-1])
+    t64 = np.interp(w, *out["fp64"])
+    t32 = np.interp(w, *out["fp32"])
+    assert abs(t64 - t32) <= 1e-3 * t64
+    assert t64 > 1.0
