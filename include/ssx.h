@@ -89,6 +89,10 @@ int ssx_tangent_rows(int dim, int n_p, const double *dphi, const double *DS, con
                      int64_t n_elem, int n_q, double r, const double *elastic, double *out, float *out32,
                      int64_t max_block_nnz, void *stream);
 
+/* SpMV row kernel (tuning knob, also used by the fused Chebyshev step): 0 = one value and one
+ * index per lane and load, 1 = aligned quadruples per lane (16-byte index and value loads) where
+ * the arrays are 16-byte aligned and a row has at least 4 lanes.  No counterpart in the reference. */
+int ssx_set_spmv_kernel(int vec4);
 /* y = alpha*A x + beta*y for CSR A (int32 indptr/indices); lanes per row: power of two <= 64.
  * Replaces PETSc MatMult (linear/solver.py:3431-3451). */
 int ssx_spmv_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, const double *x,

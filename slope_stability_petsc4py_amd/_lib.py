@@ -29,6 +29,7 @@ _SIGS: dict[str, list] = {
     "ssx_force_gather": [P, P, P, P, I64, I32, P, P],
     "ssx_tangent_rows": [I32, I32, P, P, P, P, P, P, P, P, P, I64, I64, I32, F64, P, P, P, I64, P],
     "ssx_spmv_f64": [I64, P, P, P, P, P, F64, F64, I32, P],
+    "ssx_set_spmv_kernel": [I32],
     "ssx_spmv_f32": [I64, P, P, P, P, P, F32, F32, I32, P],
     "ssx_axpby_f64": [I64, F64, P, F64, P, P],
     "ssx_axpby_f32": [I64, F32, P, F32, P, P],

@@ -42,9 +42,14 @@ def ptr(t: torch.Tensor | None) -> int | None:
 
 
 def spmv_lanes(indptr: np.ndarray) -> int:
-    """Lanes per row for the vector CSR kernel: the power of two nearest the mean row length / 4."""
+    """Lanes per row for the CSR kernels.  Rows of 48 or more entries on average (3D P2/P4
+    operators) use the aligned-quadruple rows: the power of two nearest mean / 12, between 4 and 64
+    (measured best on MI355X: 8-16 lanes for the 77-entry P2 rows and 16 for the 212-entry P4
+    rows).  Shorter rows: the smallest power of two >= mean / 4 (scalar rows below 4 lanes)."""
     n = max(int(indptr.size) - 1, 1)
     mean = float(indptr[-1]) / n
+    if mean >= 48.0:
+        return int(min(64, max(4, 2 ** round(np.log2(mean / 12.0)))))
     lanes = 1
     while lanes < 64 and lanes * 4 < mean:
         lanes *= 2

@@ -201,9 +201,17 @@ def test_spmv_vs_oracle(gpu, asm3d):
     assert _rel(y.cpu().numpy(), yo) < 1e-13
     y32 = K.matvec32(torch.from_numpy(x.astype(np.float32)).to(dev))
     assert _rel(y32.cpu().numpy().astype(np.float64), yo) < 1e-5
-    for lanes in (1, 2, 4, 8, 16, 32, 64):
-        K.lanes = lanes
-        assert _rel(K.matvec(torch.from_numpy(x).to(dev)).cpu().numpy(), yo) < 1e-13
+    # both row kernels: scalar rows, and aligned quadruples with head/tail entries by lanes 0..2
+    for vec4 in (0, 1):
+        _lib.call("ssx_set_spmv_kernel", vec4)
+        try:
+            for lanes in (1, 2, 4, 8, 16, 32, 64):
+                K.lanes = lanes
+                assert _rel(K.matvec(torch.from_numpy(x).to(dev)).cpu().numpy(), yo) < 1e-13
+                y32 = K.matvec32(torch.from_numpy(x.astype(np.float32)).to(dev))
+                assert _rel(y32.cpu().numpy().astype(np.float64), yo) < 1e-5
+        finally:
+            _lib.call("ssx_set_spmv_kernel", 1)
 
 
 @pytest.mark.parametrize("k", [1, 8, 13])

@@ -134,20 +134,24 @@ def test_fused_chebyshev_step_matches_unfused(gpu, elastic3d, use32, degree):
     rng = np.random.default_rng(degree)
     b = torch.from_numpy(rng.normal(size=K.n_rows)).to(gpu).to(dt)
     x0 = torch.from_numpy(rng.normal(size=K.n_rows)).to(gpu).to(dt)
+    from slope_stability_petsc4py_amd import _lib
     out = {}
-    for fused in (False, True):
+    for fused in (False, True, "vec4"):
         lv = _Level(K, use32)
         lv.refresh()
-        lv.fused = fused
+        lv.fused = bool(fused)
+        _lib.call("ssx_set_spmv_kernel", 1 if fused == "vec4" else 0)    # fused=True: scalar rows
         xa = torch.empty_like(b)
         lv.cheb(b, xa, degree, 0.1, zero_guess=True)
         xb = x0.clone()
         lv.cheb(b, xb, degree, 0.1, zero_guess=False)
         out[fused] = (xa.double().cpu().numpy(), xb.double().cpu().numpy())
+    _lib.call("ssx_set_spmv_kernel", 1)
     tol = 1e-5 if use32 else 1e-12
     for i in range(2):
         ref = out[False][i]
         assert np.abs(out[True][i] - ref).max() <= tol * np.abs(ref).max()
+        assert np.abs(out["vec4"][i] - ref).max() <= tol * np.abs(ref).max()
 
 
 def test_footing_limit_load_converges_to_prandtl(gpu):
