@@ -283,42 +283,50 @@ constexpr int kCBlock = 128;   // points per workgroup
 constexpr int kDStride = 37;   // LDS row stride of a point's tangent (36 + 1 against bank conflicts)
 
 // One lane per point.  The block's strains are read through LDS (one coalesced pass), the return
-// mapping writes the point's stress and tangent into LDS rows, and the block's S, DS (and the FP32
-// copy of DS) leave in coalesced passes: the block owns the contiguous ranges
-// S[p0*6 .. (p0+np)*6) and DS[p0*36 .. (p0+np)*36).
+// mapping writes the point's tangent into LDS rows, and the block's DS (and the FP32 copy of DS)
+// and then S leave in coalesced passes: the block owns the contiguous ranges S[p0*6 .. (p0+np)*6)
+// and DS[p0*36 .. (p0+np)*36).  One LDS area serves all three stagings in turn (E, then DS, then
+// S, with barriers between), which keeps a block at 38 KB of LDS: 4 blocks (8 wavefronts) per CU.
 __global__ void __launch_bounds__(kCBlock) k_constitutive_3d(const double *__restrict__ E, MatSource mat, int64_t n,
                                                              double *__restrict__ S, DSWriter w,
                                                              int8_t *__restrict__ branch) {
     __shared__ double sds[kCBlock * kDStride];
-    __shared__ double sse[kCBlock * 7];
     const int t = threadIdx.x;
     const int64_t p0 = (int64_t)blockIdx.x * kCBlock;
     const int np = (int)(n - p0 < kCBlock ? n - p0 : kCBlock);
-    for (int i = t; i < 6 * np; i += kCBlock) sse[(i / 6) * 7 + i % 6] = E[p0 * 6 + i];
+    for (int i = t; i < 6 * np; i += kCBlock) sds[(i / 6) * 7 + i % 6] = E[p0 * 6 + i];
     __syncthreads();
+    double e[6], s[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+        e[c] = t < np ? sds[t * 7 + c] : 0.0;
+        s[c] = 0.0;
+    }
+    __syncthreads();   // every lane holds its strain: the area now takes the tangent rows
     const bool want = w.on();
     if (t < np) {
         const int64_t p = p0 + t;
-        double e[6], s[6];
-#pragma unroll
-        for (int c = 0; c < 6; ++c) e[c] = sse[t * 7 + c];
         double cb, sp, G, lam;
         mat.get(p, cb, sp, G, lam);
         LdsDSWriter lw{sds + t * kDStride, want};
         const int b = mc3d_point(e, cb, sp, G, lam, s, lw);
-#pragma unroll
-        for (int c = 0; c < 6; ++c) sse[t * 7 + c] = s[c];   // own row only: no hazard with other lanes
         if (branch) branch[p] = (int8_t)b;
     }
     __syncthreads();
-    for (int i = t; i < 6 * np; i += kCBlock) S[p0 * 6 + i] = sse[(i / 6) * 7 + i % 6];
     if (want) {
         for (int i = t; i < 36 * np; i += kCBlock) {
             const double v = sds[(i / 36) * kDStride + i % 36];
             if (w.ds) w.ds[p0 * 36 + i] = v;
             if (w.ds32) w.ds32[p0 * 36 + i] = (float)v;
         }
+        __syncthreads();   // tangent rows read out: the area now takes the stresses
     }
+    if (t < np) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sds[t * 7 + c] = s[c];
+    }
+    __syncthreads();
+    for (int i = t; i < 6 * np; i += kCBlock) S[p0 * 6 + i] = sds[(i / 6) * 7 + i % 6];
 }
 
 // ---------------------------------------------------------------------------------------------
