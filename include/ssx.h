@@ -76,6 +76,11 @@ int ssx_force_elem(int dim, int n_p, const double *dphi, const double *S, const 
  * rows without slots (constrained) get 0 (fem/distributed_tangent.py:932). */
 int ssx_force_gather(const double *fe, const int32_t *row_slot_ptr, const int32_t *slot_elem, const uint8_t *slot_lrow,
                      int64_t n_rows, int nld, double *F, void *stream);
+/* Internal force, stage 2 through the node slots (ns_ptr / ns_elem / ns_lnode, element order per
+ * node): the same sums as ssx_force_gather, bit for bit, with the dim rows of a node read together;
+ * rows with row_free == 0 get 0 (fem/distributed_tangent.py:932). */
+int ssx_force_gather_nodes(int dim, int nld, const double *fe, const int32_t *ns_ptr, const int32_t *ns_elem,
+                           const uint8_t *ns_lnode, const uint8_t *row_free, int64_t n_nodes, double *F, void *stream);
 
 /* Tangent values on the fixed CSR pattern through the row-slot lists; constrained rows get a unit
  * diagonal.  With `elastic` non-NULL writes K_r = (1-r) K + r K_el; `out32` (may be NULL) receives

@@ -27,6 +27,7 @@ _SIGS: dict[str, list] = {
     "ssx_strain": [I32, I32, P, P, P, I64, I32, P, P],
     "ssx_force_elem": [I32, I32, P, P, P, I64, I32, P, P],
     "ssx_force_gather": [P, P, P, P, I64, I32, P, P],
+    "ssx_force_gather_nodes": [I32, I32, P, P, P, P, P, I64, P, P],
     "ssx_tangent_rows": [I32, I32, P, P, P, P, P, P, P, P, P, I64, I64, I32, F64, P, P, P, I64, P],
     "ssx_spmv_f64": [I64, P, P, P, P, P, F64, F64, I32, P],
     "ssx_set_spmv_kernel": [I32],

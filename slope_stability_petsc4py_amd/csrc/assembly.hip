@@ -177,6 +177,26 @@ __global__ void __launch_bounds__(256) k_force_gather(const double *__restrict__
     F[r] = acc;  // rows without slots (constrained) get 0, as the reference zeroes them
 }
 
+// Stage 2 by mesh node: lane (node, c) sums fe[e*nld + dim*lnode + c] over the node's element
+// slots (element order, the order of the row slots, so the sums are the row-slot gather's bit for
+// bit); the dim lanes of a node read dim adjacent entries of each element vector instead of one
+// scattered entry per row.  Constrained rows get 0.
+__global__ void __launch_bounds__(256) k_force_gather_nodes(const double *__restrict__ fe,
+                                                            const int32_t *__restrict__ ns_ptr,
+                                                            const int32_t *__restrict__ ns_elem,
+                                                            const uint8_t *__restrict__ ns_lnode,
+                                                            const uint8_t *__restrict__ row_free, int64_t n_nodes,
+                                                            int dim, int nld, double *__restrict__ F) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_nodes * dim) return;
+    const int64_t node = r / dim;
+    const int c = (int)(r - node * dim);
+    double acc = 0.0;
+    const int32_t s1 = ns_ptr[node + 1];
+    for (int32_t s = ns_ptr[node]; s < s1; ++s) acc += fe[(int64_t)ns_elem[s] * nld + dim * (int)ns_lnode[s] + c];
+    F[r] = row_free[r] ? acc : 0.0;
+}
+
 // ------------------------------------------------------------------------------------------
 // Tangent values on the fixed pattern.  One workgroup of ROWS lanes owns ROWS consecutive rows,
 // i.e. one contiguous run of the CSR value array; with USE_LDS that run is accumulated in LDS and
@@ -392,6 +412,17 @@ int ssx_force_gather(const double *fe, const int32_t *row_slot_ptr, const int32_
     hipLaunchKernelGGL(k_force_gather, dim3(ssx::grid_for(n_rows, 256)), dim3(256), 0, ssx::as_stream(stream), fe,
                        row_slot_ptr, slot_elem, slot_lrow, n_rows, nld, F);
     SSX_CHECK_LAUNCH("ssx_force_gather");
+    return SSX_OK;
+}
+
+int ssx_force_gather_nodes(int dim, int nld, const double *fe, const int32_t *ns_ptr, const int32_t *ns_elem,
+                           const uint8_t *ns_lnode, const uint8_t *row_free, int64_t n_nodes, double *F, void *stream) {
+    SSX_REQUIRE((dim == 2 || dim == 3) && nld > 0 && n_nodes >= 0, "ssx_force_gather_nodes: bad sizes");
+    if (n_nodes == 0) return SSX_OK;
+    SSX_REQUIRE(fe && ns_ptr && ns_elem && ns_lnode && row_free && F, "ssx_force_gather_nodes: null pointer");
+    hipLaunchKernelGGL(k_force_gather_nodes, dim3(ssx::grid_for(n_nodes * dim, 256)), dim3(256), 0,
+                       ssx::as_stream(stream), fe, ns_ptr, ns_elem, ns_lnode, row_free, n_nodes, dim, nld, F);
+    SSX_CHECK_LAUNCH("ssx_force_gather_nodes");
     return SSX_OK;
 }
 

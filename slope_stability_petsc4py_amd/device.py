@@ -135,6 +135,7 @@ class DevicePattern:
             self.ns_lnode = to_dev(ns.lnode)
             self.ns_off = to_dev(ns.off)
         self._ke = None
+        self.force_gather = "nodes"     # "nodes" (node slots, 3D) or "rows" (row slots)
 
     def new_csr(self, with32: bool = True) -> DeviceCSR:
         dev = self.indptr.device
@@ -151,8 +152,12 @@ class DevicePattern:
     def force(self, S: torch.Tensor, fe: torch.Tensor, F: torch.Tensor) -> None:
         _lib.call("ssx_force_elem", self.dim, self.n_p, ptr(self.dphi), ptr(S), ptr(self.weight), self.n_elem,
                   self.n_q, ptr(fe), stream_ptr())
-        _lib.call("ssx_force_gather", ptr(fe), ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow),
-                  self.n_rows, self.nld, ptr(F), stream_ptr())
+        if self.ns is not None and self.force_gather == "nodes":
+            _lib.call("ssx_force_gather_nodes", self.dim, self.nld, ptr(fe), ptr(self.ns_ptr), ptr(self.ns_elem),
+                      ptr(self.ns_lnode), ptr(self.row_free), self.host.n_nodes, ptr(F), stream_ptr())
+        else:
+            _lib.call("ssx_force_gather", ptr(fe), ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow),
+                      self.n_rows, self.nld, ptr(F), stream_ptr())
 
     def _ke_buffer(self, dev) -> torch.Tensor:
         if self._ke is None:

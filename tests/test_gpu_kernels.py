@@ -147,6 +147,13 @@ def test_assembly_3d_vs_oracle_and_deterministic(gpu, asm3d):
     F = torch.empty(pat.n_rows, dtype=torch.float64, device=dev)
     dp.force(torch.from_numpy(S).to(dev), fe, F)
     assert _rel(F.cpu().numpy(), oracle.force_3d_rows(geom.dphi, S, geom.weight, pat, geom.n_q)) < 1e-12
+    # the node-slot gather (default) and the row-slot gather sum in the same order: bit for bit
+    assert dp.force_gather == "nodes"
+    F_rows = torch.empty_like(F)
+    dp.force_gather = "rows"
+    dp.force(torch.from_numpy(S).to(dev), fe, F_rows)
+    dp.force_gather = "nodes"
+    assert torch.equal(F, F_rows)
     K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
     K32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev)
     DSd = torch.from_numpy(DS).to(dev)
