@@ -220,7 +220,8 @@ def run_ssr(s: Solver, *, predictor=None, store_step_u: bool = False, progress=N
                                      predictor=predictor, store_step_u=store_step_u, progress=progress)
 
 
-def run_ll(s: Solver, *, d_omega_ini_scale: float | None = None, d_t_min: float | None = None, progress=None):
+def run_ll(s: Solver, *, d_omega_ini_scale: float | None = None, d_t_min: float | None = None, progress=None,
+           predictor=None, store_step_u: bool = False):
     p = s.case.params
     if d_omega_ini_scale is None:
         d_omega_ini_scale = getattr(p, "d_omega_ini_scale", 0.2)
@@ -228,4 +229,5 @@ def run_ll(s: Solver, *, d_omega_ini_scale: float | None = None, d_t_min: float 
         d_t_min = getattr(p, "d_t_min", 1e-3)
     return ll_indirect_continuation(d_omega_ini_scale, d_t_min, p.step_max, p.omega_max_stop, p.it_newt_max,
                                     p.it_damp_max, p.tol, p.r_min, s.f, s.op, s.linear, free_mask=s.free_mask,
-                                    dim=s.case.mesh.dim, progress=progress, f_const=s.f_const)
+                                    dim=s.case.mesh.dim, progress=progress, f_const=s.f_const, predictor=predictor,
+                                    store_step_u=store_step_u)

@@ -244,6 +244,8 @@ class LLResult:
     wall_time: float
     newton_iterations: int
     linear_iterations: int
+    step_U: list = field(default_factory=list)
+    step_info: list = field(default_factory=list)
 
     @property
     def limit_load(self) -> float:
@@ -274,7 +276,8 @@ class LoadOffsetBuilder:
 
 def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_stop, it_newt_max, it_damp_max, tol,
                              r_min, f, builder, solver, *, free_mask, dim, lambda_ell: float = 1.0,
-                             progress=None, f_const: torch.Tensor | None = None) -> LLResult:
+                             progress=None, f_const: torch.Tensor | None = None, predictor=None,
+                             store_step_u: bool = False) -> LLResult:
     """Limit-load continuation in omega = f^T U (continuation/limit_load.py:66-374 with the driver's
     elastic start, cli/run_3D_hetero_SSR_capture.py:1224-1305): one elastic solve K_el U_el = f
     (strength reduced by lambda_ell), omega_el = f^T U_el, start from U = scale*U_el, t = 0, first
@@ -284,7 +287,11 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
 
     ``f_const`` (not in the reference, whose LL scales its one load): a constant load carried
     throughout, F(U) = f_const + t f.  The branch then starts from the equilibrium U_0 under f_const
-    alone (plain Newton at t = 0), omega_0 = f^T U_0, and the elastic increment is added to U_0."""
+    alone (plain Newton at t = 0), omega_0 = f^T U_0, and the elastic increment is added to U_0.
+
+    ``predictor(ctx) -> U_guess or None``: the learned-predictor slot, as in the SSR continuation
+    (ctx["states"] holds the last accepted (U, omega, t)); the guess is rescaled to the omega target
+    and used only when its first residual |t f - F(U)| is below the secant guess's (the gate)."""
     t0 = perf_counter()
     stats = NewtonStats()
     scr = _Scratch(f.device)
@@ -313,6 +320,9 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
     U_old = U.clone()
     t = 0.0
     t_hist, omega_hist, umax_hist = [0.0], [omega], [_umax(U if U_0 is None else U_0, dim)]
+    step_U = [(U_0 if U_0 is not None else torch.zeros_like(U)).clone()] if store_step_u else []
+    states = [((U_0 if U_0 is not None else torch.zeros_like(U)), omega, 0.0)]
+    step_info = []
     step = 1
     n_omega = 0
     stop_reason = "completed"
@@ -326,10 +336,36 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
             U_ini = U if den == 0.0 else U + (d_omega / den) * (U - U_old)
         else:
             U_ini = U
+        kind = "secant"
+        gate_info = {}
+        if predictor is not None and step > 1:
+            guess = predictor({"U": U, "U_old": U_old, "omega": omega, "omega_old": omega_old,
+                               "omega_target": omega_it, "lambda": t, "states": states})
+            if guess is not None:
+                d = dot(f, guess, scr)
+                if d != 0.0:
+                    guess = guess * (omega_it / d)
+                r_sec = float(torch.linalg.vector_norm(t * f - builder.build_F_reduced_vec(U_ini)).item())
+                r_l = float(torch.linalg.vector_norm(t * f - builder.build_F_reduced_vec(guess)).item())
+                gate_info = {"secant_first_residual": r_sec, "learned_first_residual": r_l}
+                if r_l < r_sec:
+                    U_ini, kind = guess, "learned"
+                else:
+                    kind = "learned_gated"
         basis_snap = solver.get_deflation_basis_snapshot()
         U_it, t_it, flag, it_newt = newton_ind_ll(U_ini, t, omega_it, it_newt_max, it_damp_max, tol, r_min, f,
                                                   builder, solver, free_mask=free_mask, stats=stats)
+        if flag != 0 and kind == "learned":
+            # the learned start failed: the secant start is tried before the step is halved
+            solver.restore_deflation_basis(basis_snap)
+            den = omega - omega_old
+            U_sec = U if den == 0.0 else U + (d_omega / den) * (U - U_old)
+            U_it, t_it, flag, it_newt = newton_ind_ll(U_sec, t, omega_it, it_newt_max, it_damp_max, tol, r_min, f,
+                                                      builder, solver, free_mask=free_mask, stats=stats)
+            kind = "secant_after_rejection"
         solver.restore_deflation_basis(basis_snap)
+        step_info.append({"step": step, "omega_target": omega_it, "flag": flag, "newton_it": it_newt,
+                          "predictor": kind, **gate_info})
         if progress is not None:
             progress({"omega_target": omega_it, "flag": flag, "newton_it": it_newt, "t": t_it})
         if flag == 1:
@@ -346,6 +382,9 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
             t_hist.append(t)
             omega_hist.append(omega)
             umax_hist.append(_umax(U, dim))
+            if store_step_u:
+                step_U.append(U.clone())
+            states = (states + [(U, omega, t)])[-3:]
             if d_t < d_t_min:
                 stop_reason = "d_t_min"
                 break
@@ -364,4 +403,4 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
         torch.cuda.synchronize()
     return LLResult(np.asarray(t_hist), np.asarray(omega_hist), np.asarray(umax_hist), U, stop_reason,
                     perf_counter() - t0, stats.newton_iterations,
-                    solver.iteration_collector.get_total_iterations() - lin0)
+                    solver.iteration_collector.get_total_iterations() - lin0, step_U, step_info)

@@ -122,9 +122,65 @@ def hetero_main(a, log):
             "newton_ratio_learned_vs_fp32": evals[best]["newton"] / max(1, evals["fp32"]["newton"]) if best else None}
 
 
+def footing_variant(seed: int, h: float, spread: float = 0.2, dphi: float = 3.0) -> cases.Case:
+    """The layered strip-footing case with its four layer materials perturbed per seed (c0 scaled
+    by U(1 - spread, 1 + spread), phi shifted by U(-dphi, dphi) degrees); seed < 0: unperturbed."""
+    base = cases.footing3d_hetero(h=h)
+    if seed < 0:
+        return base
+    rng = np.random.default_rng(seed)
+    mats = [MaterialSpec(c0=m.c0 * float(rng.uniform(1 - spread, 1 + spread)),
+                         phi=m.phi + float(rng.uniform(-dphi, dphi)), psi=m.psi, young=m.young, poisson=m.poisson,
+                         gamma_sat=m.gamma_sat, gamma_unsat=m.gamma_unsat) for m in base.materials]
+    return cases.Case(f"footing3d_seed{seed}", base.mesh, mats, base.params, footing=base.footing,
+                      gravity=base.gravity)
+
+
+def footing_main(a, log):
+    """Limit load of the layered strip footing: train on perturbed-material LL histories, evaluate
+    plain FP64, FP32 and FP32 + learned predictor on the reference materials (time to the same
+    omega, load factor there)."""
+    t0 = time.perf_counter()
+    samples, train_info = [], []
+    for seed in range(a.train):
+        case = footing_variant(seed, a.h, a.spread, a.dphi)
+        res = cases.run_ll(cases.make_solver(case, precision="fp32"), store_step_u=True)
+        info = node_info(case.mesh, case.materials)
+        samples += samples_from_history(info, res.step_U, res.omega_hist, res.t_hist)
+        train_info.append({"seed": seed, "t": res.limit_load, "newton": res.newton_iterations,
+                           "steps": len(res.t_hist), "wall": res.wall_time})
+        log("train run", train_info[-1])
+        del res
+        torch.cuda.empty_cache()
+    t_data = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    net = NodeMLP(seed=0)
+    losses = train(net, samples, epochs=a.epochs, lr=2e-3, log=lambda ep, l: log("epoch", ep, "loss", l))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t1
+    warm = footing_variant(-1, a.h)
+    warm.params.step_max = 4
+    cases.run_ll(cases.make_solver(warm, precision="fp32"))
+    evals = {}
+    for name, prec, learned in (("plain_fp64", "fp64", False), ("fp32", "fp32", False),
+                                ("learned_fp32", "fp32", True)):
+        case = footing_variant(-1, a.h)
+        pred = LearnedPredictor(net, node_info(case.mesh, case.materials), smoothing=2) if learned else None
+        res = cases.run_ll(cases.make_solver(case, precision=prec), predictor=pred)
+        evals[name] = {"t": res.limit_load, "omega": float(res.omega_hist[-1]), "stop": res.stop_reason,
+                       "wall": res.wall_time, "newton": res.newton_iterations, "linear": res.linear_iterations,
+                       "learned_steps_used": sum(1 for r in res.step_info if r["predictor"] == "learned"),
+                       "steps": len(res.t_hist)}
+        log("eval", name, evals[name])
+    return {"case": f"footing3d_hetero h={a.h}", "train_seeds": a.train, "samples": len(samples),
+            "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
+            "data_gen_s": t_data, "train_s": t_train, "train_runs": train_info, "eval": evals,
+            "speedup_learned_fp32_vs_plain_fp64": evals["plain_fp64"]["wall"] / evals["learned_fp32"]["wall"]}
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", default="homo", choices=["homo", "hetero3d_L1"])
+    ap.add_argument("--case", default="homo", choices=["homo", "hetero3d_L1", "footing3d"])
     ap.add_argument("--save", default=None, help="write the trained weights to this .npz")
     ap.add_argument("--load", default=None, help="skip training: evaluate these weights")
     ap.add_argument("--train-smoothing", type=int, default=0,
@@ -142,6 +198,9 @@ def main(argv=None):
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     if a.case == "hetero3d_L1":
         print(json.dumps(hetero_main(a, log)), flush=True)
+        return
+    if a.case == "footing3d":
+        print(json.dumps(footing_main(a, log)), flush=True)
         return
     t0 = time.perf_counter()
     samples = []

@@ -274,3 +274,45 @@ def test_lower_triangular_inverse_recursive_blocks():
         Li = lower_triangular_inverse(L, base=32)
         assert torch.allclose(Li @ L, torch.eye(n, dtype=torch.float64), atol=1e-12)
         assert torch.all(torch.triu(Li, 1) == 0)
+
+
+def _ll_tanh_case():
+    c, fval = 3.0, 1.5
+    k = torch.tensor([2.0, 2.5, 3.0, 2.2, 4.0, 2.8], dtype=torch.float64)
+    b = TanhBuilder(k=k, c=c)
+    b.U = torch.zeros(6, dtype=torch.float64)
+    f = torch.full((6,), fval, dtype=torch.float64)
+    return b, f, k, c, fval
+
+
+def test_ll_learned_predictor_slot_gate():
+    """Limit-load predictor slot: an exact branch point is used (and takes fewer Newton iterations
+    than the secant), a guess off the branch is gated; the limit load is the plain method's."""
+    b, f, k, c, fval = _ll_tanh_case()
+    args = (0.2, 1e-4, 80, 1e9, 50, 10, 1e-10, 1e-4, f, b, DiagSolver())
+    plain = continuation.ll_indirect_continuation(*args, free_mask=torch.ones(6, dtype=torch.float64), dim=3,
+                                                  store_step_u=True)
+    states = [(float(w), U.clone()) for w, U in zip(plain.omega_hist, plain.step_U)]
+    assert len(plain.step_U) == len(plain.t_hist)
+
+    def exact(ctx):
+        w = ctx["omega_target"]
+        hit = [U for (wo, U) in states if abs(wo - w) <= 1e-9 * max(1.0, abs(w))]
+        return hit[0] if hit else None
+
+    b2, *_ = _ll_tanh_case()
+    res = continuation.ll_indirect_continuation(0.2, 1e-4, 80, 1e9, 50, 10, 1e-10, 1e-4, f, b2, DiagSolver(),
+                                                free_mask=torch.ones(6, dtype=torch.float64), dim=3,
+                                                predictor=exact)
+    kinds = [r["predictor"] for r in res.step_info]
+    assert "learned" in kinds
+    assert abs(res.limit_load - plain.limit_load) < 1e-6
+    assert res.newton_iterations < plain.newton_iterations
+
+    b3, *_ = _ll_tanh_case()
+    bad = continuation.ll_indirect_continuation(
+        0.2, 1e-4, 80, 1e9, 50, 10, 1e-10, 1e-4, f, b3, DiagSolver(), free_mask=torch.ones(6, dtype=torch.float64),
+        dim=3, predictor=lambda ctx: torch.tensor([5.0, -3.0, 5.0, -3.0, 5.0, -3.0], dtype=torch.float64))
+    kinds = {r["predictor"] for r in bad.step_info}
+    assert "learned" not in kinds and "learned_gated" in kinds
+    assert abs(bad.limit_load - plain.limit_load) < 1e-12
