@@ -469,7 +469,7 @@ int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_ele
 // loads, 1 = pipelined with D from LDS, 2 = pipelined with D by scalar loads, 3 = pipelined with a
 // single LDS buffer and a 128-register budget; waves per CU of the persistent grid.
 static int ssx_ke_variant = 1;
-static int ssx_ke_waves_per_cu = 16;
+static int ssx_ke_waves_per_cu = 24;
 
 extern "C" {
 

@@ -455,7 +455,7 @@ def test_p2_element_kernel_variants_agree(gpu, variant):
                   int(ke.data_ptr()), 0)
         torch.cuda.synchronize()
     finally:
-        _lib.call("ssx_set_element_kernel", 1, 16)
+        _lib.call("ssx_set_element_kernel", 1, 24)
     exp = oracle.element_matrices_3d(dphi, (1 - r) * DS + r * Del, wt, n_e, geom.n_q)
     got = ke.cpu().numpy()
     assert np.isfinite(got).all()
