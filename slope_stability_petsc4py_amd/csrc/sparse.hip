@@ -135,7 +135,7 @@ __global__ void k_f2d(int64_t n, const float *__restrict__ x, double *__restrict
 // Arnoldi step): stage 1 is a (chunk, column) grid writing one partial per workgroup, stage 2 sums
 // each column's partials in chunk order -- deterministic, and the grid has k * n_chunk workgroups
 // instead of k.
-constexpr int kDotChunk = 4096;   // vector entries per stage-1 workgroup
+constexpr int kDotChunk = 1024;   // vector entries per stage-1 workgroup (more where n needs over kDotMaxChunks)
 constexpr int kDotMaxChunks = 64;
 
 __global__ void __launch_bounds__(256) k_dot_cols_part(int64_t n, const double *__restrict__ V, int64_t ld,
@@ -540,6 +540,9 @@ __global__ void __launch_bounds__(256) k_spmm_f64(int64_t n_rows, const int32_t 
 
 int launch_spmm(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int k,
                 const double *X, int64_t ldx, double *Y, int64_t ldy, int lanes, hipStream_t st) {
+    // `lanes` is the SpMV's choice; where that reads quadruples (about mean/12 lanes) the SpMM, whose
+    // entries each gather kc vectors, spreads a row over 4x as many lanes (about mean/3)
+    if (vec4_ok(indices, val, lanes)) lanes = lanes * 4 < 64 ? lanes * 4 : 64;
     const dim3 grid(ssx::grid_for(n_rows * lanes, 256)), block(256);
     for (int j0 = 0; j0 < k; j0 += kSpmmKC) {
         const int kc = k - j0 < kSpmmKC ? k - j0 : kSpmmKC;
