@@ -244,7 +244,9 @@ def run_ssr_leg(log_progress=True) -> dict:
     four layers, 82,815 dofs) to the factor of safety, plain FP64 GPU form (FP64 preconditioner,
     secant predictor) against the FP32-preconditioner form with the learned predictor (weights
     trained on the GPU by the project's kernels, `slope_stability_petsc4py_amd/data/`).  Wall time =
-    continuation from the first Newton solve to the last accepted step, inputs on the device."""
+    continuation from the first Newton solve to the last accepted step, inputs on the device.  The
+    faster forms also report their displacement and stress (FP64 return mapping at their final U and
+    lambda) relative to the plain FP64 form's at the same omega, and the FoS difference."""
     import torch
 
     from slope_stability_petsc4py_amd import cases
@@ -262,6 +264,8 @@ def run_ssr_leg(log_progress=True) -> dict:
     forms = [("plain_fp64", "fp64", False), ("fp32", "fp32", False)]
     if net is not None:
         forms.append(("learned_fp32", "fp32", True))
+    finals = {}
+    op64 = None
     for name, prec, learned in forms:
         s = cases.make_solver(case, precision=prec)
         pred = LearnedPredictor(net, node_info(case.mesh, case.materials)) if learned else None
@@ -270,9 +274,27 @@ def run_ssr_leg(log_progress=True) -> dict:
         out[name] = {"wall_s": round(res.wall_time, 3), "fos": res.fos, "fos_abs_err": abs(res.fos - fos_ref),
                      "newton": res.newton_iterations, "linear": res.linear_iterations,
                      "learned_steps": sum(1 for r in res.step_info if r["predictor"] == "learned")}
+        finals[name] = (res.U.clone(), float(res.fos), float(res.omega_hist[-1]))
+        if name == "plain_fp64":
+            op64 = s.op
         if log_progress:
             _log(f"ssr {name}: {out[name]}")
         del s, res
+    # errors of the faster forms against the plain FP64 form at the end of the branch (same omega):
+    # displacement, and stress evaluated by the FP64 return mapping at each form's (U, lambda)
+    U64, lam64, om64 = finals["plain_fp64"]
+    op64.reduction(lam64)
+    S64 = op64.constitutive_problem_stress(U64).clone()
+    for name in [n for n in finals if n != "plain_fp64"]:
+        U, lam, om = finals[name]
+        if abs(om - om64) > 1e-9 * abs(om64):
+            continue
+        op64.reduction(lam)
+        S = op64.constitutive_problem_stress(U)
+        out[name]["u_rel_err_vs_fp64"] = float((torch.linalg.vector_norm(U - U64) / torch.linalg.vector_norm(U64)).item())
+        out[name]["stress_rel_err_vs_fp64"] = float((torch.linalg.vector_norm(S - S64) /
+                                                     torch.linalg.vector_norm(S64)).item())
+        out[name]["fos_diff_vs_fp64"] = abs(lam - lam64)
     best = "learned_fp32" if "learned_fp32" in out else "fp32"
     out["speedup_vs_plain_fp64"] = round(out["plain_fp64"]["wall_s"] / out[best]["wall_s"], 3)
     out["fast_form"] = best

@@ -193,3 +193,28 @@ def test_footing_under_gravity_fp32_matches_fp64(gpu):
     t32 = np.interp(w, *out["fp32"])
     assert abs(t64 - t32) <= 1e-3 * t64
     assert t64 > 1.0
+
+
+def test_p4_ssr_matches_reference_history():
+    """The reference's P4 problem (SSR_hetero_ada_L1 elevated to P4, 616,322 free dofs, p-multigrid
+    P4 -> P2 -> P1, FP32 preconditioner) to omega = 6.7e6: lambda lies between the reference's
+    lambda at omega = 6.57e6 (1.566538) and at 7.0e6 (1.569076) and within 1e-3 of their linear
+    interpolation (benchmarks/slope_stability_3D_hetero_SSR_default/archive/
+    report_pmg_rank8_p2_levels_p4_omega7e6.md:72-80); the branch starts at lambda 1.0, 1.1 as the
+    reference's (lambda_init 1.0, d_lambda_init 0.1)."""
+    print("case: SSR hetero3d_L1 P4 fp32", flush=True)
+    case = cases.hetero3d_L1("P4")
+    assert case.mesh.elem.shape[0] == 35
+    case.params.omega_max_stop = 6.7e6
+    s = cases.make_solver(case, precision="fp32")
+    assert int(s.free_mask.sum().item()) == 616322
+    res = cases.run_ssr(s)
+    lam = res.fos
+    print("lambda", lam, "omega", res.omega_hist[-1], "steps", len(res.lambda_hist), "newton", res.newton_iterations,
+          "wall", res.wall_time, flush=True)
+    assert abs(res.omega_hist[-1] - 6.7e6) < 1.0
+    lo, hi = 1.566538452, 1.569076398
+    interp = lo + (6.7e6 - 6570273.222) / (7.0e6 - 6570273.222) * (hi - lo)
+    assert lo < lam < hi
+    assert abs(lam - interp) < 1e-3
+    assert abs(res.lambda_hist[0] - 1.0) < 1e-12 and abs(res.lambda_hist[1] - 1.1) < 1e-12
