@@ -409,8 +409,8 @@ def main():
                          "element blocks + row gather")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build measures one GPU only (the boxes here have one GPU); "
-              "multi-GPU sharding is design-only (see the sharding notes).", file=sys.stderr)
+        print("bench.py measures one GPU only: the path runs on a single GPU, and sharding it over several "
+              "GPUs is a design, not an implementation.", file=sys.stderr)
         sys.exit(2)
     if args.config == "p4":
         run_p4(args)

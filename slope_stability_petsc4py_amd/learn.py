@@ -12,7 +12,7 @@ element/nodal averaging over the mesh.
 Features (16 per node): normalised coordinates (3), the node's material strength (c0 scaled,
 tan phi) (2), lambda and its last increment (2), dU_{k-1}/s, dU_{k-2}/s and the secant increment
 a*dU_{k-1}/s (3 each, zero-padded in 2D).  The last layer starts at zero, so an untrained network
-is exactly the secant.  The continuation keeps plain Newton as the judge: a guess whose first
+is exactly the secant.  The continuation keeps plain Newton as the arbiter: a guess whose first
 residual is not below the secant's is rejected.
 """
 

@@ -1,5 +1,5 @@
 """Parity of the HIP kernels (called through the C-ABI) with the CPU oracle and the reference's
-golden vectors.  Tolerances: FP64 stress/tangent 1e-10 relative (north-star bar), assembly values
+golden vectors.  Tolerances: FP64 stress/tangent 1e-10 relative, assembly values
 1e-12 relative to the largest entry, integer structure bit-exact."""
 
 from pathlib import Path
