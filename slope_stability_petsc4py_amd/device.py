@@ -118,7 +118,7 @@ class DevicePattern:
         self.row_slot_ptr = to_dev(pat.row_slot_ptr)
         self.slot_elem = to_dev(pat.slot_elem)
         self.slot_lrow = to_dev(pat.slot_lrow)
-        self.slot_pos = to_dev(pat.slot_pos)
+        self._slot_pos = None      # uploaded on first use (row-slot tangent path only)
         self.elem_dofs = to_dev(pat.elem_dofs)
         self.free_idx = to_dev(pat.free_idx)
         self.dphi = to_dev(geom.dphi)
@@ -136,6 +136,12 @@ class DevicePattern:
             self.ns_off = to_dev(ns.off)
         self._ke = None
         self.force_gather = "nodes"     # "nodes" (node slots, 3D) or "rows" (row slots)
+
+    @property
+    def slot_pos(self) -> torch.Tensor:
+        if self._slot_pos is None:
+            self._slot_pos = to_dev(self.host.slot_pos)
+        return self._slot_pos
 
     def new_csr(self, with32: bool = True) -> DeviceCSR:
         dev = self.indptr.device

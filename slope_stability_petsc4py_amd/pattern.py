@@ -41,10 +41,24 @@ class OwnedPattern:
     row_slot_ptr: np.ndarray       # (n_rows+1,) int32
     slot_elem: np.ndarray          # (n_slots,) int32
     slot_lrow: np.ndarray          # (n_slots,) uint8
-    slot_pos: np.ndarray           # (n_slots, dim*n_p) int32
     elem_dofs: np.ndarray          # (n_elem, dim*n_p) int32
     free_idx: np.ndarray           # (n_free,) int64 free dofs (Fortran order of (dim, n_nodes))
     stats: dict = field(default_factory=dict)
+    # slot_pos (n_slots, dim*n_p) int32, the CSR position of every local column of every slot, is
+    # formed on first use (the node-gather path needs it for one row per node only)
+    _slot_positions: object = None
+    _slot_pos: np.ndarray | None = None
+
+    @property
+    def slot_pos(self) -> np.ndarray:
+        if self._slot_pos is None:
+            n = int(self.slot_elem.size)
+            self._slot_pos = np.ascontiguousarray(self.slot_positions(np.arange(n)).astype(np.int32))
+        return self._slot_pos
+
+    def slot_positions(self, sel: np.ndarray) -> np.ndarray:
+        """slot_pos rows of the slots `sel` (int64, -1 for constrained columns)."""
+        return self._slot_positions(np.asarray(sel, dtype=np.int64))
 
     @property
     def n_rows(self) -> int:
@@ -141,18 +155,19 @@ def build_owned_pattern(elem: np.ndarray, q_mask: np.ndarray) -> OwnedPattern:
     np.add.at(row_slot_ptr, slot_row + 1, 1)
     row_slot_ptr = np.cumsum(row_slot_ptr)
 
-    # --- slot_pos: CSR position of every local column of the slot's element ---
-    node_a = elem[slot_lrow // dim, slot_elem]              # row node per slot
-    col_nodes = elem[:, slot_elem].T                        # (n_slots, n_p)
-    pair_key = node_a[:, None] * n_nodes + col_nodes
-    pair_idx = np.searchsorted(key, pair_key)
-    start = colstart[pair_idx]                              # offset of node j's first free col in row
-    slot_pos = np.full((n_slots, nld), -1, dtype=np.int64)
-    base = indptr[slot_row][:, None]
-    for d in range(dim):
-        fr_d = q_mask[d][col_nodes]
-        val = base + start + rank[d][col_nodes]
-        slot_pos[:, d::dim] = np.where(fr_d, val, -1)
+    # --- slot_pos: CSR position of every local column of the slot's element (on demand) ---
+    def slot_positions(sel: np.ndarray) -> np.ndarray:
+        s_row, s_elem, s_lrow = slot_row[sel], slot_elem[sel], slot_lrow[sel].astype(np.int64)
+        node_a = elem[s_lrow // dim, s_elem]                  # row node per slot
+        col_nodes = elem[:, s_elem].T                           # (n_sel, n_p)
+        pair_idx = np.searchsorted(key, node_a[:, None] * n_nodes + col_nodes)
+        start = colstart[pair_idx]                              # offset of node j's first free col in row
+        pos = np.full((sel.size, nld), -1, dtype=np.int64)
+        base = indptr[s_row][:, None]
+        for d in range(dim):
+            fr_d = q_mask[d][col_nodes]
+            pos[:, d::dim] = np.where(fr_d, base + start + rank[d][col_nodes], -1)
+        return pos
 
     return OwnedPattern(
         dim=dim, n_p=n_p, n_nodes=n_nodes,
@@ -160,10 +175,11 @@ def build_owned_pattern(elem: np.ndarray, q_mask: np.ndarray) -> OwnedPattern:
         row_free=free_dof.astype(np.uint8),
         constrained_diag_positions=constrained_diag.astype(np.int32),
         row_slot_ptr=row_slot_ptr.astype(np.int32), slot_elem=slot_elem.astype(np.int32),
-        slot_lrow=slot_lrow.astype(np.uint8), slot_pos=np.ascontiguousarray(slot_pos.astype(np.int32)),
+        slot_lrow=slot_lrow.astype(np.uint8),
         elem_dofs=np.ascontiguousarray(edofs.astype(np.int32)),
         free_idx=np.flatnonzero(q_mask.reshape(-1, order="F")).astype(np.int64),
         stats={"n_slots": float(n_slots), "nnz": float(nnz)},
+        _slot_positions=slot_positions,
     )
 
 
@@ -199,7 +215,7 @@ def node_slots(pat: OwnedPattern) -> NodeSlots:
     _check_int32("node_slot_count", n)
     sel = np.repeat(rsp[r0] - ptr[:-1], cnt) + np.arange(n)                    # row-slot index per node slot
     base = np.repeat(ip[r0], cnt)
-    pos = pat.slot_pos[sel].astype(np.int64)
+    pos = pat.slot_positions(sel)
     off = np.where(pos >= 0, pos - base[:, None], -1)
     row_len = np.diff(ip)
     max_len = int(row_len[pat.row_free.astype(bool)].max()) if pat.row_free.any() else 1
