@@ -8,7 +8,12 @@ left/right edge and apex) at every point, the geometry (dphi, w) and the fixed C
 One step = the Newton-iteration hot path on those inputs:
   return mapping (E -> S, DS)  +  internal force (S -> F)  +  tangent assembly into the fixed
   pattern with the regularised operator K_r = (1-r)K + r K_el and its FP32 copy.
-`value` = algorithmic HBM bytes of the step / step time (GB/s).
+`value` = algorithmic HBM bytes of the step / step time (GB/s).  `roofline` is the slowest kernel
+of the step (the tangent) against the 8 TB/s HBM peak, timed with HIP events on its launch stream,
+with the HBM bytes per launch read by rocprofv3 (profiles/traffic.json).  `cpu_baseline` is the
+reference's own C kernels (built from its sources into oracle/_ref, OpenMP on the host cores) on
+the same arrays.  `ssr` is the end-to-end leg of the metric: the reference's 3D heterogeneous P2
+SSR to the factor of safety, plain FP64 against the FP32 and learned + FP32 forms.
 
 Prints one JSON line.  Rank/world handling: one GPU only (--gpus > 1 exits with a message).
 """
