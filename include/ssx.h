@@ -234,6 +234,14 @@ int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *ind
                        const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c, double *Ac,
                        void *stream);
 
+/* Sparse Galerkin coarse operator onto a given coarse CSR pattern (c_indptr, c_indices sorted per
+ * row): c_val (caller-zeroed) += P^T A P at the positions the pattern holds, products outside it
+ * dropped (for nested P1-in-P2 spaces they sum to zero).  FP64 atomics, setup-time only.  Replaces
+ * the same PETSc MatPtAP (linear/solver.py:510-513) for coarse levels too large for a dense matrix. */
+int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                     const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, const int32_t *c_indptr,
+                     const int32_t *c_indices, double *c_val, void *stream);
+
 /* Coarse solve y = Ainv x of the p-multigrid with the explicit inverse of the symmetric coarse
  * operator kept as the upper triangle of 64 x 64 tiles (half the bytes of a dense product per
  * apply; deterministic).  Replaces the coarse solve of the manual V-cycle (linear/solver.py:97-160,

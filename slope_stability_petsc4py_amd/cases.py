@@ -182,12 +182,14 @@ class Solver:
 
 def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_refresh: int = 0,
                 coarse_precision: str | None = None, refresh_iterations: int = 20,
-                refresh_ratio: float = 1.5) -> Solver:
+                refresh_ratio: float = 1.5, coarse: str = "auto") -> Solver:
     """precision 'fp64': FP64 preconditioner; 'fp32': FP32 tangent shadow + FP32 preconditioner
     (the outer FGMRES, residuals and return mapping stay FP64).  `coarse_refresh`: the p-multigrid
     coarse operator is rebuilt every that many preconditioner setups (1 = every Newton iteration),
     or, with 0 (default), when a solve needs more than `refresh_iterations` FGMRES iterations and
-    more than `refresh_ratio` times the iterations of the first solve after the last rebuild."""
+    more than `refresh_ratio` times the iterations of the first solve after the last rebuild.
+    `coarse`: 'dense' (dense P1 level), 'aggregate' (sparse P1 level above a dense aggregate level)
+    or 'auto' (aggregate when the free P1 dofs exceed 50000)."""
     use32 = precision == "fp32"
     op = ConstitutiveOperator.from_mesh(case.mesh, case.materials, Davis_type=case.davis, fp32_shadow=use32)
     f_field = op.gravity_load(case.mesh)
@@ -205,7 +207,7 @@ def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_
     elif pc == "pmg":
         prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32, coarse_refresh=coarse_refresh,
                           coarse_precision=coarse_precision, refresh_iterations=refresh_iterations,
-                          refresh_ratio=refresh_ratio)
+                          refresh_ratio=refresh_ratio, coarse=coarse, coord=case.mesh.coord)
     else:
         prec = ChebyshevJacobi(use32=use32)
     lin = DeflatedFGMRES(prec, tolerance=case.params.linear_tol, max_iterations=case.params.linear_maxit)

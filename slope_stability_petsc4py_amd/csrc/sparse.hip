@@ -487,6 +487,58 @@ extern "C" int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const i
 }
 
 // ---------------------------------------------------------------------------------------------
+// Galerkin coarse operator onto a given coarse CSR pattern: Ac[I][J] += p_iI a_ij p_jJ for every
+// (I, J) the pattern holds (columns sorted within a row; binary search per product).  Products
+// that fall outside the pattern are dropped: for nested spaces (P1 inside P2 on one mesh, P the
+// interpolation) P^T A P is the coarse stiffness matrix, whose sparsity is the coarse node
+// adjacency, so those products sum to zero up to rounding.  FP64 atomics, as in the dense form;
+// the caller zeroes c_val first.
+// ---------------------------------------------------------------------------------------------
+namespace {
+__global__ void k_galerkin_csr(int64_t n_rows, const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+                               const double *__restrict__ val, const int32_t *__restrict__ p_ptr,
+                               const int32_t *__restrict__ p_idx, const double *__restrict__ p_w,
+                               const int32_t *__restrict__ c_indptr, const int32_t *__restrict__ c_indices,
+                               double *__restrict__ c_val) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
+    if (row >= n_rows) return;
+    const int32_t pi0 = p_ptr[row], pi1 = p_ptr[row + 1];
+    if (pi0 == pi1) return;
+    for (int32_t k = indptr[row] + threadIdx.x; k < indptr[row + 1]; k += blockDim.x) {
+        const int32_t col = indices[k];
+        const double a = val[k];
+        const int32_t pj0 = p_ptr[col], pj1 = p_ptr[col + 1];
+        for (int32_t s = pi0; s < pi1; ++s) {
+            const double ai = a * p_w[s];
+            const int32_t I = p_idx[s];
+            const int32_t r0 = c_indptr[I], r1 = c_indptr[I + 1];
+            for (int32_t t = pj0; t < pj1; ++t) {
+                const int32_t J = p_idx[t];
+                int32_t lo = r0, hi = r1;     // first position with c_indices[pos] >= J
+                while (lo < hi) {
+                    const int32_t mid = lo + ((hi - lo) >> 1);
+                    if (c_indices[mid] < J) lo = mid + 1; else hi = mid;
+                }
+                if (lo < r1 && c_indices[lo] == J) atomicAdd(&c_val[lo], ai * p_w[t]);
+            }
+        }
+    }
+}
+}  // namespace
+
+extern "C" int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                                const int32_t *p_ptr, const int32_t *p_idx, const double *p_w,
+                                const int32_t *c_indptr, const int32_t *c_indices, double *c_val, void *stream) {
+    SSX_REQUIRE(n_rows >= 0, "ssx_galerkin_csr: bad sizes");
+    if (n_rows == 0) return SSX_OK;
+    dim3 block(16, 16);
+    hipLaunchKernelGGL(k_galerkin_csr, dim3(ssx::grid_for(n_rows, 16)), block, 0, ssx::as_stream(stream), n_rows,
+                       indptr, indices, val, p_ptr, p_idx, p_w, c_indptr, c_indices, c_val);
+    SSX_CHECK_LAUNCH("ssx_galerkin_csr");
+    return SSX_OK;
+}
+
+// ---------------------------------------------------------------------------------------------
 // Multi-vector SpMV and the A-orthonormalisation of the deflation basis.
 //
 // SpMM: Y_j = A X_j for k vectors stored as rows (X_j = X + j*ldx); a group of G lanes per row as

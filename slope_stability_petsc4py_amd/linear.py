@@ -18,7 +18,9 @@ Preconditioners
   * ``PMultigrid``: V-cycle P2 -> P1 (vertex dofs), Chebyshev(Jacobi) smoothing in FP32
     (values32) or FP64, Galerkin coarse operator P^T A P formed on the device, inverted densely
     (FP64 Cholesky, explicit inverse; a coarse solve is one matrix-vector product).  Replaces
-    Hypre BoomerAMG / the manual p-MG shell (linear/solver.py:96-779).
+    Hypre BoomerAMG / the manual p-MG shell (linear/solver.py:96-779).  Above a size limit the P1
+    level stays sparse (Galerkin product on the P1 pattern, Chebyshev smoothing) and the dense
+    level below it holds rigid-body modes on node aggregates.
   * ``ChebyshevJacobi``: polynomial preconditioner alone.
 """
 
@@ -159,8 +161,9 @@ class _Level:
         self.xb = torch.empty(n, dtype=self.dtype, device=dev)    # second iterate buffer (fused steps)
         self.lmax = 2.0
         self.fused = True
+        self._v = None      # last power-iteration iterate (normalised), the next refresh's start
 
-    def refresh(self, power_its: int = 12, seed: int = 0):
+    def refresh(self, power_its: int = 12, seed: int = 0, warm_its: int = 4):
         A = self.A
         fn = "ssx_inv_diag_f32" if self.use32 else "ssx_inv_diag_f64"
         vals = A.values32 if self.use32 else A.values
@@ -168,13 +171,19 @@ class _Level:
         # power iteration on D^-1 A without per-step normalisation (the iterate grows by at most the
         # spectral radius per step, far inside the FP32 range over a dozen steps): the estimate is
         # |w_k| / |w_{k-1}|, the same quotient the normalised iteration forms, from two norms and one
-        # host read
-        g = torch.Generator(device="cpu").manual_seed(seed)
-        v = torch.rand(self.n, generator=g, dtype=torch.float64).to(self.dinv.device).to(self.dtype)
-        v /= torch.linalg.vector_norm(v)
+        # host read.  The first refresh starts from a seeded random vector; later ones start from
+        # the previous refresh's last iterate (the operator changes little between Newton steps)
+        # and take `warm_its` steps
         if power_its < 1:
             self.lmax = 1.0
             return
+        if self._v is None:
+            g = torch.Generator(device="cpu").manual_seed(seed)
+            v = torch.rand(self.n, generator=g, dtype=torch.float64).to(self.dinv.device).to(self.dtype)
+            v /= torch.linalg.vector_norm(v)
+        else:
+            v = self._v
+            power_its = max(1, min(power_its, warm_its))
         src, dst = v, self.r
         for _ in range(power_its - 1):
             spmv(A, src, dst, use32=self.use32)
@@ -183,9 +192,11 @@ class _Level:
         prev = torch.linalg.vector_norm(src)
         spmv(A, src, dst, use32=self.use32)
         dst.mul_(self.dinv)
-        ratio = torch.linalg.vector_norm(dst) / torch.clamp(prev, min=torch.finfo(self.dtype).tiny)
+        nrm = torch.linalg.vector_norm(dst)
+        ratio = nrm / torch.clamp(prev, min=torch.finfo(self.dtype).tiny)
         lam = float(ratio.item())
         self.lmax = 1.1 * lam if lam > 0.0 and np.isfinite(lam) else 1.0
+        self._v = (dst / nrm) if lam > 0.0 and np.isfinite(lam) else None
 
     def cheb(self, b: torch.Tensor, x: torch.Tensor, degree: int, lo_frac: float, zero_guess: bool):
         """`degree` Chebyshev(Jacobi) steps on A x = b over [lo_frac*lmax, lmax]."""
@@ -458,6 +469,93 @@ class PMultigridP4:
             z.copy_(x)
 
 
+def rbm_aggregates(coord: np.ndarray, elem: np.ndarray, free: np.ndarray, radius: int = 2):
+    """Tentative prolongator of rigid-body modes on node aggregates (unsmoothed aggregation).
+
+    Aggregates: every node whose graph neighbourhood of the given radius (node adjacency of the
+    elements `elem`, vertex numbering) is still unassigned becomes a root and takes that whole
+    neighbourhood; the remaining nodes then join the aggregate most of their neighbours belong to,
+    sweep by sweep.  Columns: per aggregate, an orthonormal basis (SVD, relative cut 1e-10) of the
+    rigid-body modes (dim translations and dim(dim-1)/2 rotations about the aggregate's centroid)
+    restricted to the free dofs (dof = dim*node + comp), so constrained dofs get empty rows and a
+    partly constrained aggregate keeps only the modes its free dofs can carry.
+
+    Returns (t_ptr, t_idx, t_w, n_t): the CSR rows of T over the dim*n_nodes dofs."""
+    import scipy.sparse as sp
+    coord = np.asarray(coord, dtype=np.float64)
+    elem = np.asarray(elem, dtype=np.int64)
+    free = np.asarray(free, dtype=bool)
+    dim, n = coord.shape
+    nv = elem.shape[0]
+    ri = np.repeat(elem.T, nv, axis=1).reshape(-1)
+    ci = np.tile(elem.T, (1, nv)).reshape(-1)
+    G = sp.csr_matrix((np.ones(ri.size, dtype=np.float32), (ri, ci)), shape=(n, n))
+    G.sum_duplicates()
+    G.data[:] = 1.0
+    H = G
+    for _ in range(max(1, int(radius)) - 1):
+        H = H @ G
+        H.data[:] = 1.0
+    agg = np.full(n, -1, dtype=np.int64)
+    na = 0
+    hp, hi = H.indptr, H.indices
+    for i in range(n):
+        if agg[i] >= 0:
+            continue
+        nb = hi[hp[i]:hp[i + 1]]
+        if (agg[nb] < 0).all():
+            agg[nb] = na
+            na += 1
+    gp, gi = G.indptr, G.indices
+    while (agg < 0).any():
+        left = np.flatnonzero(agg < 0)
+        new = agg.copy()
+        for i in left:
+            a = agg[gi[gp[i]:gp[i + 1]]]
+            a = a[a >= 0]
+            if a.size:
+                new[i] = np.bincount(a).argmax()
+        if (new == agg).all():           # nodes with no assigned neighbour: aggregates of their own
+            new[left] = na + np.arange(left.size)
+            na += left.size
+        agg = new
+    order = np.argsort(agg, kind="stable")
+    starts = np.searchsorted(agg[order], np.arange(na + 1))
+    n_rot = dim * (dim - 1) // 2
+    m = dim + n_rot
+    rows, cols, vals = [], [], []
+    n_t = 0
+    for a in range(na):
+        nodes = order[starts[a]:starts[a + 1]]
+        d = coord[:, nodes] - coord[:, nodes].mean(axis=1, keepdims=True)
+        R = np.zeros((nodes.size, dim, m))
+        for c in range(dim):
+            R[:, c, c] = 1.0
+        if dim == 3:
+            R[:, 1, 3], R[:, 2, 3] = -d[2], d[1]      # about x
+            R[:, 0, 4], R[:, 2, 4] = d[2], -d[0]      # about y
+            R[:, 0, 5], R[:, 1, 5] = -d[1], d[0]      # about z
+        else:
+            R[:, 0, 2], R[:, 1, 2] = -d[1], d[0]
+        dofs = (dim * nodes[:, None] + np.arange(dim)).reshape(-1)
+        f = free[dofs]
+        if not f.any():
+            continue
+        U, s, _ = np.linalg.svd(R.reshape(-1, m)[f], full_matrices=False)
+        k = int((s > 1e-10 * s[0]).sum()) if s[0] > 0 else 0
+        fd = dofs[f]
+        rows.append(np.repeat(fd, k))
+        cols.append(np.tile(n_t + np.arange(k), fd.size))
+        vals.append(U[:, :k].reshape(-1))
+        n_t += k
+    rows = np.concatenate(rows) if rows else np.zeros(0, dtype=np.int64)
+    cols = np.concatenate(cols) if cols else np.zeros(0, dtype=np.int64)
+    vals = np.concatenate(vals) if vals else np.zeros(0)
+    T = sp.csr_matrix((vals, (rows, cols)), shape=(dim * n, n_t))
+    T.sort_indices()
+    return T.indptr.astype(np.int64), T.indices.astype(np.int64), T.data.astype(np.float64), n_t
+
+
 class PMultigrid:
     """Two-level V-cycle: Chebyshev(Jacobi) pre/post smoothing on the P2 operator, Galerkin coarse
     operator on the P1 vertex dofs.
@@ -467,11 +565,18 @@ class PMultigrid:
     matrix-vector product: bandwidth-bound, instead of two latency-bound triangular solves.  The
     coarse operator is rebuilt on the first setup and then every `coarse_refresh` setups (the fine
     level always uses the current operator; FGMRES is flexible, so a lagged coarse correction
-    only costs iterations, never correctness)."""
+    only costs iterations, never correctness).
+
+    When the free P1 dofs exceed `coarse_dense_max` (coarse="auto"; the dense matrix grows as its
+    square) or with coarse="aggregate", the P1 level is kept sparse instead: the Galerkin product on
+    the P1 pattern (`ssx_galerkin_csr`, formed at every setup), Chebyshev smoothing of degree
+    `degree1`, and below it the dense level of rigid-body modes on node aggregates
+    (`rbm_aggregates`), inverted and refreshed as above."""
 
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
                  coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 20,
-                 refresh_ratio: float = 1.5):
+                 refresh_ratio: float = 1.5, coarse: str = "auto", coord=None, coarse_dense_max: int = 50000,
+                 degree1: int = 4):
         self.use32 = use32
         self.degree = degree
         self.lo_frac = lo_frac
@@ -490,19 +595,46 @@ class PMultigrid:
                         "coarse_fp64_fallbacks": 0}
         self._setups = 0
         p_ptr, p_idx, p_w, n_c, c_free = p2_to_p1_prolongation(elem, q_mask)
-        # the coarse space is numbered over the free coarse dofs only: no column of P touches a
-        # constrained coarse dof, so the dense coarse system (factorised and inverted at each
-        # rebuild, applied every Krylov iteration) carries no identity block
-        keep = np.flatnonzero(c_free)
-        remap = np.full(n_c, -1, dtype=np.int64)
-        remap[keep] = np.arange(keep.size)
-        p_idx = remap[p_idx]
-        if p_idx.size and p_idx.min() < 0:
-            raise AssertionError("P2 -> P1 interpolation references a constrained coarse dof")
-        self.n_c = int(keep.size)
-        self.c_free = np.ones(self.n_c, dtype=bool)
+        n_free_c = int(c_free.sum())
+        if coarse not in ("auto", "dense", "aggregate"):
+            raise ValueError(f"coarse must be 'auto', 'dense' or 'aggregate', not {coarse!r}")
+        if coarse == "auto":
+            coarse = "aggregate" if n_free_c > coarse_dense_max else "dense"
+        if coarse == "aggregate" and coord is None:
+            raise ValueError("the aggregation level needs the node coordinates")
+        self.coarse = coarse
+        self.degree1 = int(degree1)
+        if coarse == "dense":
+            # the coarse space is numbered over the free coarse dofs only: no column of P touches a
+            # constrained coarse dof, so the dense coarse system (factorised and inverted at each
+            # rebuild, applied every Krylov iteration) carries no identity block
+            keep = np.flatnonzero(c_free)
+            remap = np.full(n_c, -1, dtype=np.int64)
+            remap[keep] = np.arange(keep.size)
+            p_idx = remap[p_idx]
+            if p_idx.size and p_idx.min() < 0:
+                raise AssertionError("P2 -> P1 interpolation references a constrained coarse dof")
+            self.n_c = int(keep.size)
+            self.c_free = np.ones(self.n_c, dtype=bool)
+            self.n_d = self.n_c
+        else:
+            # sparse P1 level (the Galerkin product on the P1 pattern, unit rows on constrained
+            # dofs, Chebyshev smoothing) above a dense level of rigid-body modes on aggregates
+            from .pattern import build_owned_pattern
+            elem = np.asarray(elem, dtype=np.int64)
+            dim = q_mask.shape[0]
+            verts = np.unique(elem[: dim + 1].reshape(-1))
+            cnum = np.full(q_mask.shape[1], -1, dtype=np.int64)
+            cnum[verts] = np.arange(verts.size)
+            elem_c = cnum[elem[: dim + 1]]
+            self.c_pat = build_owned_pattern(elem_c, q_mask[:, verts])
+            self.n_c = int(n_c)
+            self.c_free = c_free
+            self._t_host = rbm_aggregates(np.asarray(coord, dtype=np.float64)[:, verts], elem_c, c_free)
+            self.n_d = int(self._t_host[3])
         self._p_host = (p_ptr, p_idx, p_w)
         self.level = None
+        self.level1 = None
         self.Ainv = None          # packed symmetric tiles of the coarse inverse
         self._tile_part = None
         self._dev_ready = False
@@ -522,10 +654,34 @@ class PMultigrid:
         self.p_ptr = torch.from_numpy(p_ptr.astype(np.int32)).to(dev)
         self.p_idx = torch.from_numpy(p_idx.astype(np.int32)).to(dev)
         self.p_w = torch.from_numpy(p_w).to(dev)
-        self.cfix = torch.from_numpy(np.flatnonzero(~self.c_free)).to(dev)
         dt = torch.float32 if self.use32 else torch.float64
         self.rc = torch.empty(self.n_c, dtype=dt, device=dev)
         self.xc = torch.empty(self.n_c, dtype=dt, device=dev)
+        if self.coarse == "dense":
+            # the dense level is the P1 level: Galerkin product of the fine operator through P
+            self.d_ptr, self.d_idx, self.d_w = self.p_ptr, self.p_idx, self.p_w
+            self.cfix = torch.from_numpy(np.flatnonzero(~self.c_free)).to(dev)
+        else:
+            cp = self.c_pat
+            self.Kc = DeviceCSR(self.n_c, torch.from_numpy(cp.indptr).to(dev), torch.from_numpy(cp.indices).to(dev),
+                                torch.zeros(cp.nnz, dtype=torch.float64, device=dev),
+                                torch.zeros(cp.nnz, dtype=torch.float32, device=dev), spmv_lanes(cp.indptr))
+            self.kc_cdiag = torch.from_numpy(cp.constrained_diag_positions.astype(np.int64)).to(dev)
+            t_ptr, t_idx, t_w, n_t = self._t_host
+            T = sp.csr_matrix((t_w, t_idx, t_ptr), shape=(self.n_c, n_t))
+            self.T = DeviceCSR(self.n_c, *mk(T), None, spmv_lanes(T.indptr))
+            self.T.values32 = self.T.values.to(torch.float32)
+            TT = T.T.tocsr()
+            self.TT = DeviceCSR(n_t, *mk(TT), None, spmv_lanes(TT.indptr))
+            self.TT.values32 = self.TT.values.to(torch.float32)
+            # the dense level is the aggregate level: Galerkin product of the P1 level through T
+            self.d_ptr = torch.from_numpy(t_ptr.astype(np.int32)).to(dev)
+            self.d_idx = torch.from_numpy(t_idx.astype(np.int32)).to(dev)
+            self.d_w = torch.from_numpy(t_w).to(dev)
+            self.cfix = torch.zeros(0, dtype=torch.int64, device=dev)
+            self.res1 = torch.empty(self.n_c, dtype=dt, device=dev)
+            self.r2 = torch.empty(n_t, dtype=dt, device=dev)
+            self.x2 = torch.empty(n_t, dtype=dt, device=dev)
         self._dev_ready = True
 
     def setup(self, A: DeviceCSR, mesh_info=None):
@@ -537,10 +693,26 @@ class PMultigrid:
         t0 = perf_counter()
         self.level.refresh()
         self.timings["level_refresh"] += perf_counter() - t0
+        dense_src = A
+        if self.coarse == "aggregate":
+            # the sparse P1 level follows the current operator at every setup
+            t0 = perf_counter()
+            Kc = self.Kc
+            Kc.values.zero_()
+            _lib.call("ssx_galerkin_csr", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.p_ptr),
+                      ptr(self.p_idx), ptr(self.p_w), ptr(Kc.indptr), ptr(Kc.indices), ptr(Kc.values), _st())
+            Kc.values[self.kc_cdiag] = 1.0
+            _lib.call("ssx_cast_f64_f32", Kc.nnz, ptr(Kc.values), ptr(Kc.values32), _st())
+            if self.level1 is None:
+                self.level1 = _Level(Kc, self.use32)
+            self.level1.refresh()
+            torch.cuda.synchronize()
+            self.timings["p1_level"] = self.timings.get("p1_level", 0.0) + perf_counter() - t0
+            dense_src = Kc
         due = self.coarse_refresh > 0 and self._setups % self.coarse_refresh == 0
         if self.Ainv is None or due or self._stale:
             self._fresh_its = None
-            self._coarse(A, dev)
+            self._coarse(dense_src, dev)
             self._stale = False
         self._setups += 1
 
@@ -561,9 +733,9 @@ class PMultigrid:
         tm = self.timings
         t0 = perf_counter()
         self.Ainv = None
-        Ac = torch.zeros((self.n_c, self.n_c), dtype=torch.float64, device=dev)
-        _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.p_ptr),
-                  ptr(self.p_idx), ptr(self.p_w), self.n_c, ptr(Ac), _st())
+        Ac = torch.zeros((self.n_d, self.n_d), dtype=torch.float64, device=dev)
+        _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.d_ptr),
+                  ptr(self.d_idx), ptr(self.d_w), self.n_d, ptr(Ac), _st())
         Ac[self.cfix, self.cfix] = 1.0
         torch.cuda.synchronize()
         t1 = perf_counter()
@@ -610,11 +782,11 @@ class PMultigrid:
             del Linv
         # the inverse is kept as the upper triangle of 64 x 64 tiles in the level precision
         dt = torch.float32 if self.use32 else torch.float64
-        te, pe = sym_tile_sizes(self.n_c)
+        te, pe = sym_tile_sizes(self.n_d)
         if self._tile_part is None or self._tile_part.dtype != dt:
             self._tile_part = torch.empty(pe, dtype=dt, device=dev)
         self.Ainv = torch.empty(te, dtype=dt, device=dev)
-        _lib.call("ssx_sym_tiles_pack", self.n_c, ptr(Ainv), int(Ainv.dtype == torch.float64), ptr(self.Ainv),
+        _lib.call("ssx_sym_tiles_pack", self.n_d, ptr(Ainv), int(Ainv.dtype == torch.float64), ptr(self.Ainv),
                   int(dt == torch.float64), _st())
         del Ainv
         torch.cuda.synchronize()
@@ -629,10 +801,23 @@ class PMultigrid:
         axpby(1.0, b, 0.0, res)
         spmv(Lv.A, x, res, alpha=-1.0, beta=1.0, use32=self.use32)          # res = b - A x
         spmv(self.PT, res, self.rc, use32=self.use32)                          # restrict
-        _lib.call("ssx_sym_tiles_gemv_f32" if self.use32 else "ssx_sym_tiles_gemv_f64", self.n_c,  # coarse solve
-                  ptr(self.Ainv), ptr(self.rc), ptr(self._tile_part), ptr(self.xc), _st())
+        if self.coarse == "dense":
+            self._dense_solve(self.rc, self.xc)                                 # coarse solve
+        else:
+            L1 = self.level1                                                    # P1 level V-cycle
+            L1.cheb(self.rc, self.xc, self.degree1, self.lo_frac, zero_guess=True)
+            axpby(1.0, self.rc, 0.0, self.res1)
+            spmv(L1.A, self.xc, self.res1, alpha=-1.0, beta=1.0, use32=self.use32)
+            spmv(self.TT, self.res1, self.r2, use32=self.use32)
+            self._dense_solve(self.r2, self.x2)
+            spmv(self.T, self.x2, self.xc, alpha=1.0, beta=1.0, use32=self.use32)
+            L1.cheb(self.rc, self.xc, self.degree1, self.lo_frac, zero_guess=False)
         spmv(self.P, self.xc, x, alpha=1.0, beta=1.0, use32=self.use32)       # prolong + correct
         Lv.cheb(b, x, self.degree, self.lo_frac, zero_guess=False)          # post-smooth
+
+    def _dense_solve(self, b: torch.Tensor, x: torch.Tensor) -> None:
+        _lib.call("ssx_sym_tiles_gemv_f32" if self.use32 else "ssx_sym_tiles_gemv_f64", self.n_d,
+                  ptr(self.Ainv), ptr(b), ptr(self._tile_part), ptr(x), _st())
 
     def apply(self, r: torch.Tensor, z: torch.Tensor) -> None:
         Lv = self.level

@@ -38,6 +38,9 @@ def main(argv=None):
                     help="FGMRES iterations of one solve above which the coarse operator is rebuilt")
     ap.add_argument("--coarse-precision", default=None, choices=["fp32", "fp64"],
                     help="coarse factorisation precision (default: the preconditioner precision)")
+    ap.add_argument("--coarse", default="auto", choices=["auto", "dense", "aggregate"],
+                    help="p-multigrid coarse level: dense P1, or sparse P1 above a dense aggregate level "
+                         "(auto: aggregate above 50000 free P1 dofs)")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
@@ -64,7 +67,8 @@ def main(argv=None):
     if a.omega_max:
         case.params.omega_max_stop = a.omega_max
     s = cases.make_solver(case, precision=a.precision, pc=a.pc, coarse_refresh=a.coarse_refresh,
-                          coarse_precision=a.coarse_precision, refresh_iterations=a.refresh_iterations)
+                          coarse_precision=a.coarse_precision, refresh_iterations=a.refresh_iterations,
+                          coarse=a.coarse)
     torch.cuda.synchronize()
     t_setup = time.perf_counter() - t0
 
@@ -83,6 +87,7 @@ def main(argv=None):
                "n_dof": int(s.op.pattern.n_rows), "n_elem": int(case.mesh.n_elem),
                "coarse_refresh": a.coarse_refresh, "linear_solver": s.linear.iteration_collector.summary(),
                "pc_timings": getattr(s.linear.pc, "timings", None),
+               "coarse": getattr(s.linear.pc, "coarse", None), "n_coarse_dense": getattr(s.linear.pc, "n_d", None),
                "timings": s.op.get_total_time()}
     else:
         res = cases.run_ll(s, progress=prog)
@@ -92,7 +97,8 @@ def main(argv=None):
                "stop_reason": res.stop_reason, "newton_iterations": res.newton_iterations,
                "linear_iterations": res.linear_iterations, "n_dof": int(s.op.pattern.n_rows),
                "n_elem": int(case.mesh.n_elem), "linear_solver": s.linear.iteration_collector.summary(),
-               "pc_timings": getattr(s.linear.pc, "timings", None), "extra": case.extra}
+               "pc_timings": getattr(s.linear.pc, "timings", None), "extra": case.extra,
+               "coarse": getattr(s.linear.pc, "coarse", None), "n_coarse_dense": getattr(s.linear.pc, "n_d", None)}
     print(json.dumps(out), flush=True)
 
 

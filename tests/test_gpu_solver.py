@@ -26,11 +26,18 @@ def elastic3d():
 
 
 @pytest.mark.parametrize("use32", [False, True])
-@pytest.mark.parametrize("pc", ["pmg", "cheb"])
+@pytest.mark.parametrize("pc", ["pmg", "pmg_agg", "cheb"])
 def test_fgmres_solves_elastic_system(gpu, elastic3d, use32, pc):
     print(f"case: fgmres pc={pc} fp32={use32}", flush=True)
     m, op, K, f = elastic3d
-    prec = PMultigrid(m.elem, m.q_mask, use32=use32) if pc == "pmg" else ChebyshevJacobi(use32=use32)
+    if pc == "pmg":
+        prec = PMultigrid(m.elem, m.q_mask, use32=use32)
+        assert prec.coarse == "dense"
+    elif pc == "pmg_agg":
+        # sparse P1 level + dense rigid-body-mode aggregate level
+        prec = PMultigrid(m.elem, m.q_mask, use32=use32, coarse="aggregate", coord=m.coord)
+    else:
+        prec = ChebyshevJacobi(use32=use32)
     solver = DeflatedFGMRES(prec, tolerance=1e-8, max_iterations=300)
     solver.setup_preconditioner(K)
     solver.A_orthogonalize(K)
@@ -43,6 +50,35 @@ def test_fgmres_solves_elastic_system(gpu, elastic3d, use32, pc):
     its = solver.iteration_collector.iterations[-1]
     if pc == "pmg":
         assert its < 60, its
+    if pc == "pmg_agg":
+        assert its < 120, its
+
+
+def test_sparse_galerkin_matches_dense_on_the_p1_pattern(gpu, elastic3d):
+    """P^T A P on the P1 pattern (ssx_galerkin_csr) against the dense product (ssx_galerkin_dense):
+    equal at the pattern's positions, and the dense product vanishes (to rounding) outside them."""
+    m, op, K, f = elastic3d
+    prec = PMultigrid(m.elem, m.q_mask, use32=False, coarse="aggregate", coord=m.coord)
+    prec.setup(K)
+    Kc = prec.Kc.to_scipy().toarray()
+    from slope_stability_petsc4py_amd import _lib
+    from slope_stability_petsc4py_amd.device import ptr, stream_ptr
+    n_c = prec.n_c
+    Ad = torch.zeros((n_c, n_c), dtype=torch.float64, device=gpu)
+    _lib.call("ssx_galerkin_dense", K.n_rows, ptr(K.indptr), ptr(K.indices), ptr(K.values), ptr(prec.p_ptr),
+              ptr(prec.p_idx), ptr(prec.p_w), n_c, ptr(Ad), stream_ptr())
+    D = Ad.cpu().numpy().T          # column-major fill
+    fix = np.flatnonzero(~prec.c_free)
+    D[fix, fix] = 1.0
+    mask = prec.Kc.to_scipy().toarray() != 0
+    pat = np.zeros_like(mask)
+    cp = prec.c_pat
+    for i in range(n_c):
+        pat[i, cp.indices[cp.indptr[i]:cp.indptr[i + 1]]] = True
+    scale = np.abs(D).max()
+    assert np.abs(Kc - np.where(pat, D, 0.0)).max() <= 1e-12 * scale
+    assert np.abs(D[~pat]).max() <= 1e-12 * scale
+    assert mask.sum() > 0 and np.allclose(Kc, Kc.T, rtol=0, atol=1e-12 * scale)
 
 
 def test_deflation_basis_reuses_previous_solutions(gpu, elastic3d):
@@ -178,11 +214,13 @@ def test_footing_under_gravity_fp32_matches_fp64(gpu):
     """Limit load of a strip footing on the layered slope under its own weight (constant gravity
     load, F(U) = f_g + t f): the FP32-preconditioner form follows the plain FP64 form's load path
     -- t at equal omega within 1e-3 -- and the branch starts from the gravity equilibrium
-    (omega_0 = f^T U_g > 0, t = 0)."""
+    (omega_0 = f^T U_g > 0, t = 0).  The FP32 form with the sparse P1 level above the dense
+    aggregate level (the coarse form of large meshes) follows the same path."""
     print("case: footing3d_hetero h=2", flush=True)
     out = {}
-    for prec in ("fp64", "fp32"):
-        res = cases.run_ll(cases.make_solver(cases.footing3d_hetero(h=2.0), precision=prec))
+    for prec in ("fp64", "fp32", "fp32_agg"):
+        coarse = "aggregate" if prec.endswith("_agg") else "dense"
+        res = cases.run_ll(cases.make_solver(cases.footing3d_hetero(h=2.0), precision=prec[:4], coarse=coarse))
         out[prec] = (np.asarray(res.omega_hist), np.asarray(res.t_hist))
         print(prec, "t", res.t_hist[-1], "omega", res.omega_hist[-1], "steps", len(res.t_hist), "newton",
               res.newton_iterations, "wall", res.wall_time, flush=True)
@@ -193,6 +231,8 @@ def test_footing_under_gravity_fp32_matches_fp64(gpu):
     t32 = np.interp(w, *out["fp32"])
     assert abs(t64 - t32) <= 1e-3 * t64
     assert t64 > 1.0
+    w = min(out["fp64"][0][-1], out["fp32_agg"][0][-1])
+    assert abs(np.interp(w, *out["fp64"]) - np.interp(w, *out["fp32_agg"])) <= 1e-3 * t64
 
 
 def test_p4_ssr_matches_reference_history():
