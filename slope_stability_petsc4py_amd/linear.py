@@ -576,7 +576,7 @@ class PMultigrid:
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
                  coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 20,
                  refresh_ratio: float = 1.5, coarse: str = "auto", coord=None, coarse_dense_max: int = 50000,
-                 degree1: int = 4):
+                 degree1: int = 12):
         self.use32 = use32
         self.degree = degree
         self.lo_frac = lo_frac
