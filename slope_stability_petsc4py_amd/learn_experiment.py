@@ -162,17 +162,21 @@ def footing_main(a, log):
     warm.params.step_max = 4
     cases.run_ll(cases.make_solver(warm, precision="fp32"))
     evals = {}
+    eh = a.eval_h or a.h
     for name, prec, learned in (("plain_fp64", "fp64", False), ("fp32", "fp32", False),
                                 ("learned_fp32", "fp32", True)):
-        case = footing_variant(-1, a.h)
+        case = footing_variant(-1, eh)
+        if a.eval_omega:
+            case.params.omega_max_stop = a.eval_omega
         pred = LearnedPredictor(net, node_info(case.mesh, case.materials), smoothing=2) if learned else None
-        res = cases.run_ll(cases.make_solver(case, precision=prec), predictor=pred)
+        res = cases.run_ll(cases.make_solver(case, precision=prec), predictor=pred,
+                           progress=lambda ev: log("step", name, ev.get("t"), ev.get("newton_it")))
         evals[name] = {"t": res.limit_load, "omega": float(res.omega_hist[-1]), "stop": res.stop_reason,
                        "wall": res.wall_time, "newton": res.newton_iterations, "linear": res.linear_iterations,
                        "learned_steps_used": sum(1 for r in res.step_info if r["predictor"] == "learned"),
                        "steps": len(res.t_hist)}
         log("eval", name, evals[name])
-    return {"case": f"footing3d_hetero h={a.h}", "train_seeds": a.train, "samples": len(samples),
+    return {"case": f"footing3d_hetero train h={a.h}, eval h={eh}", "train_seeds": a.train, "samples": len(samples),
             "loss_first": losses[0] if losses else None, "loss_last": losses[-1] if losses else None,
             "data_gen_s": t_data, "train_s": t_train, "train_runs": train_info, "eval": evals,
             "speedup_learned_fp32_vs_plain_fp64": evals["plain_fp64"]["wall"] / evals["learned_fp32"]["wall"]}
@@ -193,6 +197,8 @@ def main(argv=None):
     ap.add_argument("--spread", type=float, default=0.2, help="hetero3d_L1 training: relative spread of c0")
     ap.add_argument("--dphi", type=float, default=3.0, help="hetero3d_L1 training: spread of phi (degrees)")
     ap.add_argument("--h", type=float, default=2.0)
+    ap.add_argument("--eval-h", type=float, default=None, help="footing3d: evaluate on this mesh size (default --h)")
+    ap.add_argument("--eval-omega", type=float, default=None, help="footing3d: evaluation runs stop at this omega")
     ap.add_argument("--nz", type=int, default=6)
     a = ap.parse_args(argv)
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
