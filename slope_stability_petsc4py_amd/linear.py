@@ -576,8 +576,11 @@ class PMultigrid:
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
                  coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 20,
                  refresh_ratio: float = 1.5, coarse: str = "auto", coord=None, coarse_dense_max: int = 50000,
-                 degree1: int = 12):
+                 degree1: int = 12, p1_refresh: str = "every"):
         self.use32 = use32
+        if p1_refresh not in ("every", "stale"):
+            raise ValueError(f"p1_refresh must be 'every' or 'stale', not {p1_refresh!r}")
+        self.p1_refresh = p1_refresh
         self.degree = degree
         self.lo_frac = lo_frac
         # coarse_refresh N >= 1: rebuild the coarse operator every N setups; 0: rebuild only when a
@@ -694,8 +697,11 @@ class PMultigrid:
         self.level.refresh()
         self.timings["level_refresh"] += perf_counter() - t0
         dense_src = A
-        if self.coarse == "aggregate":
-            # the sparse P1 level follows the current operator at every setup
+        due = self.coarse_refresh > 0 and self._setups % self.coarse_refresh == 0
+        rebuild = self.Ainv is None or due or self._stale
+        if self.coarse == "aggregate" and (self.p1_refresh == "every" or rebuild):
+            # the sparse P1 level follows the current operator at every setup ("every"), or is
+            # rebuilt with the dense level below it ("stale")
             t0 = perf_counter()
             Kc = self.Kc
             Kc.values.zero_()
@@ -708,9 +714,9 @@ class PMultigrid:
             self.level1.refresh()
             torch.cuda.synchronize()
             self.timings["p1_level"] = self.timings.get("p1_level", 0.0) + perf_counter() - t0
-            dense_src = Kc
-        due = self.coarse_refresh > 0 and self._setups % self.coarse_refresh == 0
-        if self.Ainv is None or due or self._stale:
+        if self.coarse == "aggregate":
+            dense_src = self.Kc
+        if rebuild:
             self._fresh_its = None
             self._coarse(dense_src, dev)
             self._stale = False
