@@ -16,10 +16,13 @@ Pinning summary (what checks each part beyond retyping):
   davis_reduction   closed forms: for psi = 0 Davis B gives tan(phi_l) = sin(phi_1), phi_1 =
                     atan(tan(phi)/lambda), c_bar = 2 cos(phi_1) (c0/lambda) cos(phi_l); at c0 = 6,
                     phi = 45 deg, lambda = 1: c_bar = 4 sqrt(3), sin_phi = 1/sqrt(3).
-  mc2d              elastic / apex closed forms, yield-surface consistency of returned stresses,
-                    DS vs central differences of S, and the reference's end-to-end 2D SSR
-                    history (FoS) through the full solver -- element-level parity against the
-                    reference's NumPy is otherwise unpinned (the reference cannot be imported here).
+  mc2d              the closest-point projection of the trial stress onto the six Mohr-Coulomb
+                    planes (exact active-set enumeration, no sort and no branch formulas) over
+                    all five branches and the tie cases of the eigenvalue sort; elastic (with
+                    sigma_33) and apex closed forms; yield surface and edge equalities; DS vs
+                    central differences; planted mistakes (``_mutation``) that these checks catch
+                    (tests/test_oracle_mc2d.py); end to end, the reference's 2D SSR history.
+  spmv              scipy CSR product (tests/test_oracle_mc2d.py).
   strain/force      brute-force B from the reference's strain-operator rule (fem/assembly.py:203-230).
   pattern           set-based restatement of fem/distributed_tangent.py:440-478 and :369-437.
 """
@@ -240,9 +243,17 @@ def davis_reduction(c0, phi, psi, lam, davis_type="B"):
     raise ValueError("Incorrect choice of the Davis approach")
 
 
-def mc2d(E, c_bar, sin_phi, shear, lame):
+def mc2d(E, c_bar, sin_phi, shear, lame, *, _mutation: str | None = None):
     """Plane-strain return mapping, loop form of constitutive/problem.py:143-475.
-    E (n, 3) -> S (n, 4), DS (n, 9) column-major 3x3, branch (n,).  Small inputs only."""
+    E (n, 3) -> S (n, 4), DS (n, 9) column-major 3x3, branch (n,).  Small inputs only.
+
+    ``_mutation`` plants one named mistake (tests only: it shows that the pinning checks of
+    tests/test_oracle_mc2d.py fail on it): "sort_strict" (`>` for `>=` in the sort test of :224),
+    "no_sort" (the eps33-between swap of :224-230 dropped), "denom_lr" (denom_l and denom_r of
+    :250-253 exchanged), "sign_s3" (sign of the (1 - sin phi) term of sigma_3 on the smooth face,
+    :281), "shear_sign" (sign of the in-plane shear projection, :175), "no_s33" (sigma_33 of the
+    elastic branch dropped, :270), "mat3_half" (factor 2 of mat3 on the smooth face dropped,
+    :289)."""
     E = np.asarray(E, dtype=np.float64)
     n = E.shape[0]
     S = np.zeros((n, 4))
@@ -260,6 +271,8 @@ def mc2d(E, c_bar, sin_phi, shear, lame):
             Da = np.zeros(9)
         else:
             Pa = np.array([(ex - a2) / I2, (ey - a2) / I2, (g / 2.0) / I2, 0.0])
+            if _mutation == "shear_sign":
+                Pa[2] = -Pa[2]
         Pb = np.array([1.0, 1.0, 0.0, 0.0]) - Pa
         if I2 != 0.0:
             x, y = Pa[:3], Pb[:3]
@@ -271,7 +284,7 @@ def mc2d(E, c_bar, sin_phi, shear, lame):
         Pc = np.array([0.0, 0.0, 0.0, 1.0])
         eigs = [(a1, Pa, Da), (a2, Pb, -Da), (a3, Pc, np.zeros(9))]
         e1, e2, e3 = eigs[0], eigs[1], eigs[2]
-        if (a1 >= a3) and (a3 > a2):
+        if ((a1 > a3) if _mutation == "sort_strict" else (a1 >= a3)) and (a3 > a2) and _mutation != "no_sort":
             e2, e3 = eigs[2], eigs[1]
         if a3 > a1:
             e1, e2, e3 = eigs[2], eigs[0], eigs[1]
@@ -283,22 +296,25 @@ def mc2d(E, c_bar, sin_phi, shear, lame):
         ds_ = 4 * lam * sp ** 2 + 2 * G * (1 + sp) ** 2 + 2 * G * (1 - sp) ** 2
         dl_ = 4 * lam * sp ** 2 + G * (1 + sp) ** 2 + 2 * G * (1 - sp) ** 2
         dr_ = 4 * lam * sp ** 2 + 2 * G * (1 + sp) ** 2 + G * (1 - sp) ** 2
+        if _mutation == "denom_lr":
+            dl_, dr_ = dr_, dl_
         ls = f / ds_
         ll = (G * ((1 + sp) * (l1 + l2) - 2 * (1 - sp) * l3) + 2 * lam * sp * tr - cb) / dl_
         lr = (G * (2 * (1 + sp) * l1 - (1 - sp) * (l2 + l3)) + 2 * lam * sp * tr - cb) / dr_
         vol = np.outer(v3, v3)
         if f <= 0:
-            S[p] = [lam * I1 + 2 * G * ex, lam * I1 + 2 * G * ey, G * g, lam * I1]
+            S[p] = [lam * I1 + 2 * G * ex, lam * I1 + 2 * G * ey, G * g, 0.0 if _mutation == "no_s33" else lam * I1]
             DS[p] = (vol * lam + 2 * np.diag([1.0, 1.0, 0.5]) * G).reshape(-1, order="F")
             br[p] = 0
         elif ls <= min(gsl, gsr):
             s1 = lam * tr + 2 * G * l1 - ls * (2 * lam * sp + 2 * G * (1 + sp))
             s2 = lam * tr + 2 * G * l2 - ls * (2 * lam * sp)
-            s3 = lam * tr + 2 * G * l3 - ls * (2 * lam * sp - 2 * G * (1 - sp))
+            s3 = lam * tr + 2 * G * l3 - ls * (2 * lam * sp + (2 if _mutation == "sign_s3" else -2) * G * (1 - sp))
             S[p] = s1 * P1 + s2 * P2 + s3 * P3
             q6 = 2 * G * (1 + sp) * P1[:3] - 2 * G * (1 - sp) * P3[:3] + v3 * 2 * lam * sp
             M = (s1 * D1 + s2 * D2 + s3 * D3).reshape(3, 3, order="F") + vol * lam \
-                + 2 * G * (np.outer(P1[:3], P1[:3]) + np.outer(P2[:3], P2[:3]) + np.outer(P3[:3], P3[:3])) \
+                + (1 if _mutation == "mat3_half" else 2) * G * np.outer(P1[:3], P1[:3]) \
+                + 2 * G * (np.outer(P2[:3], P2[:3]) + np.outer(P3[:3], P3[:3])) \
                 - np.outer(q6, q6) / ds_
             DS[p] = M.reshape(-1, order="F")
             br[p] = 1
