@@ -227,20 +227,26 @@ int ssx_bf16_to_f32(int64_t n, const void *x, float *y, void *stream);
 int ssx_mse_f32(int64_t n, const float *p, const float *t, const float *mask, float inv_count, float *g,
                 float *loss_part, void *stream);
 
-/* Dense Galerkin coarse operator Ac (n_c x n_c, column-major, caller-zeroed) += P^T A P with P given
- * as CSR rows (p_ptr, p_idx, p_w) over the fine rows of A; FP64 atomics, setup-time only.
+/* Dense Galerkin coarse operator Ac (n_c x n_c, column-major, caller-zeroed) = P^T A P with P given
+ * as CSR rows (p_ptr, p_idx, p_w) over the fine rows of A (nnz entries).  The products are summed in
+ * 64-bit fixed point with integer atomics, so the result is bitwise the same at every call; the
+ * scale is 2^(61 - ceil(log2(max|a| * w_bound))), where w_bound must bound sum_ij |p_iI| |p_jJ| over
+ * the products of any one coarse entry (e.g. the largest column sum of |P| times the longest row of
+ * A times max|p|).  work: 8 bytes of device scratch.  Setup-time work.
  * Replaces the PETSc MatPtAP of the p-multigrid set-up (linear/solver.py:510-513). */
-int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
-                       const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c, double *Ac,
-                       void *stream);
+int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int64_t nnz,
+                       const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c, double w_bound,
+                       double *Ac, void *work, void *stream);
 
 /* Sparse Galerkin coarse operator onto a given coarse CSR pattern (c_indptr, c_indices sorted per
- * row): c_val (caller-zeroed) += P^T A P at the positions the pattern holds, products outside it
- * dropped (for nested P1-in-P2 spaces they sum to zero).  FP64 atomics, setup-time only.  Replaces
- * the same PETSc MatPtAP (linear/solver.py:510-513) for coarse levels too large for a dense matrix. */
-int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+ * row, c_nnz entries): c_val (caller-zeroed) = P^T A P at the positions the pattern holds, products
+ * outside it dropped (for nested P1-in-P2 spaces they sum to zero).  Fixed-point sums as in
+ * ssx_galerkin_dense (bitwise repeatable).  Replaces the same PETSc MatPtAP (linear/solver.py:510-513)
+ * for coarse levels too large for a dense matrix. */
+int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int64_t nnz,
                      const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, const int32_t *c_indptr,
-                     const int32_t *c_indices, double *c_val, void *stream);
+                     const int32_t *c_indices, int64_t c_nnz, double w_bound, double *c_val, void *work,
+                     void *stream);
 
 /* Coarse solve y = Ainv x of the p-multigrid with the explicit inverse of the symmetric coarse
  * operator kept as the upper triangle of 64 x 64 tiles (half the bytes of a dense product per

@@ -51,8 +51,8 @@ _SIGS: dict[str, list] = {
     "ssx_cheb_update_f64": [I64, P, P, F64, F64, P, P, P],
     "ssx_inv_diag_f32": [I64, I64, P, P, P, P, P],
     "ssx_inv_diag_f64": [I64, I64, P, P, P, P, P],
-    "ssx_galerkin_dense": [I64, P, P, P, P, P, P, I64, P, P],
-    "ssx_galerkin_csr": [I64, P, P, P, P, P, P, P, P, P, P],
+    "ssx_galerkin_dense": [I64, P, P, P, I64, P, P, P, I64, F64, P, P, P],
+    "ssx_galerkin_csr": [I64, P, P, P, I64, P, P, P, P, P, I64, F64, P, P, P],
     "ssx_element_matrices_p4": [P, P, P, I64, F64, P, P, P, P],
     "ssx_element_matrices_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P],
     "ssx_set_element_kernel": [I32, I32],

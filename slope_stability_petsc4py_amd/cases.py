@@ -203,7 +203,7 @@ def make_solver(case: Case, *, precision: str = "fp64", pc: str = "pmg", coarse_
     free_mask = to_dev(case.mesh.q_mask.T.reshape(-1).astype(np.float64))
     if pc == "pmg" and case.mesh.elem_type == "P4":
         prec = PMultigridP4(case.mesh, op, use32=use32, coarse_refresh=coarse_refresh,
-                            refresh_iterations=refresh_iterations, refresh_ratio=refresh_ratio)
+                            refresh_iterations=refresh_iterations, refresh_ratio=refresh_ratio, coarse=coarse)
     elif pc == "pmg":
         prec = PMultigrid(case.mesh.elem, case.mesh.q_mask, use32=use32, coarse_refresh=coarse_refresh,
                           coarse_precision=coarse_precision, refresh_iterations=refresh_iterations,

@@ -265,7 +265,10 @@ class ConstitutiveOperator:
         F = self._force().index_select(0, self.pattern.free_idx)
         return F.cpu().numpy() if getattr(self, "_kind", "") == "numpy" else F
 
-    build_F_free_local = build_F_free
+    def build_F_free_local(self):
+        """Internal force at the owned free rows (constitutive/problem.py:1997-2010).  One device
+        owns every row, so the owned free rows are all free rows, in free-dof order."""
+        return self.build_F_free()
 
     # flat full-length device vectors (constrained dofs zero) for the device Newton solvers
     def build_F_vec(self) -> torch.Tensor:
@@ -329,7 +332,12 @@ class ConstitutiveOperator:
         self.constitutive_problem_stress(U)
         return self.build_F_free()
 
-    build_F_all_free_local = build_F_all_free
+    def build_F_all_free_local(self, lam, U):
+        """reduction(lam), stress at U, force at the owned free rows (constitutive/problem.py:
+        2045-2048); probed by the Newton damping, nonlinear/damping.py:181-182."""
+        self.reduction(lam)
+        self.constitutive_problem_stress(U)
+        return self.build_F_free_local()
 
     def build_F_K_tangent_all(self, lam, U):
         self.reduction(lam)
@@ -353,7 +361,11 @@ class ConstitutiveOperator:
         self.constitutive_problem_stress(U)
         return self.build_F_free()
 
-    build_F_reduced_free_local = build_F_reduced_free
+    def build_F_reduced_free_local(self, U):
+        """Stress at U and force at the owned free rows (constitutive/problem.py:2077-2079);
+        probed by the line search, nonlinear/damping.py:105-106."""
+        self.constitutive_problem_stress(U)
+        return self.build_F_free_local()
 
     def build_F_K_tangent_reduced(self, U):
         self.constitutive_problem_stress_tangent(U)

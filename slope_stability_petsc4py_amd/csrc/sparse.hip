@@ -446,20 +446,61 @@ int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 }  // extern "C"
 
 // ---------------------------------------------------------------------------------------------
-// Galerkin coarse operator for the p-multigrid: Ac = P^T A P accumulated into a dense column-major
-// (n_c x n_c) matrix.  P has at most a few entries per fine row (p_ptr/p_idx/p_w, CSR); one lane
-// per nonzero of A scatters its (<= 4) products with FP64 atomics.  Setup-time work of the
-// preconditioner (replaces the PETSc MatPtAP of linear/solver.py:510-513 feeding the coarse solve).
+// Galerkin coarse operators of the p-multigrid, Ac = P^T A P (replaces the PETSc MatPtAP of
+// linear/solver.py:510-513 feeding the coarse solve).  One lane per nonzero a_ij of the fine
+// operator scatters its products p_iI a_ij p_jJ (P has a few entries per fine row: p_ptr/p_idx/
+// p_w, CSR) into the coarse matrix.
+//
+// The scatter sums in 64-bit fixed point: each product is rounded to an integer multiple of 2^-f
+// and added with an integer atomic.  Integer addition is associative, so every coarse entry is the
+// same bit pattern whatever order the atomics land in (FP64 atomics give a different rounding at
+// every run).  The scale comes from max|a| (an atomicMax on the IEEE bits of |a|, which order the
+// same way as the values, also order-independent) and a bound `w_bound` on the weights supplied by
+// the caller: every partial sum of a coarse entry is at most sum |p_iI| |a_ij| |p_jJ| <= max|a| *
+// w_bound, and f is chosen so that this stays below 2^61.  The resolution, max|a| w_bound 2^-61,
+// is far below the FP64 rounding of the large entries.
 // ---------------------------------------------------------------------------------------------
 namespace {
+using u64 = unsigned long long;
+
+__global__ void k_zero_u64(u64 *w) { w[0] = 0ull; }
+
+__global__ void k_absmax_bits(int64_t n, const double *__restrict__ val, u64 *__restrict__ amax) {
+    u64 m = 0ull;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const u64 b = (u64)__double_as_longlong(fabs(val[i]));
+        m = b > m ? b : m;
+    }
+    // wave maximum, then one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        const u64 o = __shfl_down(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(amax, m);
+}
+
+// 2^f with max|a| * w_bound * 2^f < 2^61 (f may be negative); 1 when A is zero.
+__device__ __forceinline__ double fixed_scale(const u64 *amax, double w_bound) {
+    const double bound = __longlong_as_double((long long)amax[0]) * w_bound;
+    if (!(bound > 0.0)) return 1.0;
+    int e;
+    frexp(bound, &e);   // bound < 2^e
+    return ldexp(1.0, 61 - e);
+}
+
+__device__ __forceinline__ void add_fixed(u64 *dst, double v, double scale) {
+    atomicAdd(dst, (u64)__double2ll_rn(v * scale));
+}
+
 __global__ void k_galerkin_dense(int64_t n_rows, const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
                                  const double *__restrict__ val, const int32_t *__restrict__ p_ptr,
                                  const int32_t *__restrict__ p_idx, const double *__restrict__ p_w, int64_t n_c,
-                                 double *__restrict__ Ac) {
+                                 const u64 *__restrict__ amax, double w_bound, u64 *__restrict__ Ac) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
     if (row >= n_rows) return;
     const int32_t pi0 = p_ptr[row], pi1 = p_ptr[row + 1];
     if (pi0 == pi1) return;
+    const double scale = fixed_scale(amax, w_bound);
     for (int32_t k = indptr[row] + threadIdx.x; k < indptr[row + 1]; k += blockDim.x) {
         const int32_t col = indices[k];
         const double a = val[k];
@@ -467,43 +508,25 @@ __global__ void k_galerkin_dense(int64_t n_rows, const int32_t *__restrict__ ind
         for (int32_t s = pi0; s < pi1; ++s) {
             const double ai = a * p_w[s];
             const int64_t I = p_idx[s];
-            for (int32_t t = pj0; t < pj1; ++t)
-                atomicAdd(&Ac[(int64_t)p_idx[t] * n_c + I], ai * p_w[t]);
+            for (int32_t t = pj0; t < pj1; ++t) add_fixed(&Ac[(int64_t)p_idx[t] * n_c + I], ai * p_w[t], scale);
         }
     }
 }
-}  // namespace
 
-extern "C" int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
-                                  const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c,
-                                  double *Ac, void *stream) {
-    SSX_REQUIRE(n_rows >= 0 && n_c >= 0, "ssx_galerkin_dense: bad sizes");
-    if (n_rows == 0) return SSX_OK;
-    dim3 block(16, 16);
-    hipLaunchKernelGGL(k_galerkin_dense, dim3(ssx::grid_for(n_rows, 16)), block, 0, ssx::as_stream(stream), n_rows,
-                       indptr, indices, val, p_ptr, p_idx, p_w, n_c, Ac);
-    SSX_CHECK_LAUNCH("ssx_galerkin_dense");
-    return SSX_OK;
-}
-
-// ---------------------------------------------------------------------------------------------
-// Galerkin coarse operator onto a given coarse CSR pattern: Ac[I][J] += p_iI a_ij p_jJ for every
-// (I, J) the pattern holds (columns sorted within a row; binary search per product).  Products
-// that fall outside the pattern are dropped: for nested spaces (P1 inside P2 on one mesh, P the
-// interpolation) P^T A P is the coarse stiffness matrix, whose sparsity is the coarse node
-// adjacency, so those products sum to zero up to rounding.  FP64 atomics, as in the dense form;
-// the caller zeroes c_val first.
-// ---------------------------------------------------------------------------------------------
-namespace {
+// Coarse operator onto a given coarse CSR pattern (columns sorted within a row; binary search per
+// product).  Products outside the pattern are dropped: for nested spaces (P1 inside P2 on one
+// mesh, P the interpolation) P^T A P is the coarse stiffness matrix, whose sparsity is the coarse
+// node adjacency, so those products sum to zero up to rounding.
 __global__ void k_galerkin_csr(int64_t n_rows, const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
                                const double *__restrict__ val, const int32_t *__restrict__ p_ptr,
                                const int32_t *__restrict__ p_idx, const double *__restrict__ p_w,
                                const int32_t *__restrict__ c_indptr, const int32_t *__restrict__ c_indices,
-                               double *__restrict__ c_val) {
+                               const u64 *__restrict__ amax, double w_bound, u64 *__restrict__ c_val) {
     const int64_t row = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
     if (row >= n_rows) return;
     const int32_t pi0 = p_ptr[row], pi1 = p_ptr[row + 1];
     if (pi0 == pi1) return;
+    const double scale = fixed_scale(amax, w_bound);
     for (int32_t k = indptr[row] + threadIdx.x; k < indptr[row + 1]; k += blockDim.x) {
         const int32_t col = indices[k];
         const double a = val[k];
@@ -519,21 +542,65 @@ __global__ void k_galerkin_csr(int64_t n_rows, const int32_t *__restrict__ indpt
                     const int32_t mid = lo + ((hi - lo) >> 1);
                     if (c_indices[mid] < J) lo = mid + 1; else hi = mid;
                 }
-                if (lo < r1 && c_indices[lo] == J) atomicAdd(&c_val[lo], ai * p_w[t]);
+                if (lo < r1 && c_indices[lo] == J) add_fixed(&c_val[lo], ai * p_w[t], scale);
             }
         }
     }
 }
+
+// In place: fixed-point sums -> FP64.
+__global__ void k_fixed_to_f64(int64_t n, u64 *__restrict__ buf, const u64 *__restrict__ amax, double w_bound) {
+    const double inv = 1.0 / fixed_scale(amax, w_bound);   // a power of two: exact
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const long long q = (long long)buf[i];
+        reinterpret_cast<double *>(buf)[i] = (double)q * inv;
+    }
+}
+
+void galerkin_prologue(int64_t nnz, const double *val, u64 *amax, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, amax);
+    const int64_t want = (nnz + 255) / 256;
+    const unsigned grid = (unsigned)(want < 4096 ? (want > 0 ? want : 1) : 4096);
+    hipLaunchKernelGGL(k_absmax_bits, dim3(grid), dim3(256), 0, st, nnz, val, amax);
+}
 }  // namespace
 
-extern "C" int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
-                                const int32_t *p_ptr, const int32_t *p_idx, const double *p_w,
-                                const int32_t *c_indptr, const int32_t *c_indices, double *c_val, void *stream) {
-    SSX_REQUIRE(n_rows >= 0, "ssx_galerkin_csr: bad sizes");
-    if (n_rows == 0) return SSX_OK;
+extern "C" int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                                  int64_t nnz, const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, int64_t n_c,
+                                  double w_bound, double *Ac, void *work, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && n_c >= 0 && nnz >= 0, "ssx_galerkin_dense: bad sizes");
+    SSX_REQUIRE(w_bound > 0.0, "ssx_galerkin_dense: w_bound must be positive");
+    if (n_rows == 0 || n_c == 0) return SSX_OK;
+    SSX_REQUIRE(indptr && indices && val && p_ptr && p_idx && p_w && Ac && work, "ssx_galerkin_dense: null pointer");
+    hipStream_t st = ssx::as_stream(stream);
+    u64 *amax = reinterpret_cast<u64 *>(work);
+    galerkin_prologue(nnz, val, amax, st);
     dim3 block(16, 16);
-    hipLaunchKernelGGL(k_galerkin_csr, dim3(ssx::grid_for(n_rows, 16)), block, 0, ssx::as_stream(stream), n_rows,
-                       indptr, indices, val, p_ptr, p_idx, p_w, c_indptr, c_indices, c_val);
+    hipLaunchKernelGGL(k_galerkin_dense, dim3(ssx::grid_for(n_rows, 16)), block, 0, st, n_rows, indptr, indices, val,
+                       p_ptr, p_idx, p_w, n_c, amax, w_bound, reinterpret_cast<u64 *>(Ac));
+    hipLaunchKernelGGL(k_fixed_to_f64, dim3(ssx::grid_for(n_c * n_c < 4096 * 256 ? n_c * n_c : 4096 * 256, 256)),
+                       dim3(256), 0, st, n_c * n_c, reinterpret_cast<u64 *>(Ac), amax, w_bound);
+    SSX_CHECK_LAUNCH("ssx_galerkin_dense");
+    return SSX_OK;
+}
+
+extern "C" int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
+                                int64_t nnz, const int32_t *p_ptr, const int32_t *p_idx, const double *p_w,
+                                const int32_t *c_indptr, const int32_t *c_indices, int64_t c_nnz, double w_bound,
+                                double *c_val, void *work, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && c_nnz >= 0 && nnz >= 0, "ssx_galerkin_csr: bad sizes");
+    SSX_REQUIRE(w_bound > 0.0, "ssx_galerkin_csr: w_bound must be positive");
+    if (n_rows == 0 || c_nnz == 0) return SSX_OK;
+    SSX_REQUIRE(indptr && indices && val && p_ptr && p_idx && p_w && c_indptr && c_indices && c_val && work,
+                "ssx_galerkin_csr: null pointer");
+    hipStream_t st = ssx::as_stream(stream);
+    u64 *amax = reinterpret_cast<u64 *>(work);
+    galerkin_prologue(nnz, val, amax, st);
+    dim3 block(16, 16);
+    hipLaunchKernelGGL(k_galerkin_csr, dim3(ssx::grid_for(n_rows, 16)), block, 0, st, n_rows, indptr, indices, val,
+                       p_ptr, p_idx, p_w, c_indptr, c_indices, amax, w_bound, reinterpret_cast<u64 *>(c_val));
+    hipLaunchKernelGGL(k_fixed_to_f64, dim3(ssx::grid_for(c_nnz < 4096 * 256 ? c_nnz : 4096 * 256, 256)), dim3(256),
+                       0, st, c_nnz, reinterpret_cast<u64 *>(c_val), amax, w_bound);
     SSX_CHECK_LAUNCH("ssx_galerkin_csr");
     return SSX_OK;
 }

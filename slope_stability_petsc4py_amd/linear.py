@@ -400,6 +400,17 @@ class _Transfer:
         self.p_w = torch.from_numpy(np.asarray(p_w, dtype=np.float64)).to(dev)
 
 
+def galerkin_weight_bound(p_ptr: np.ndarray, p_idx: np.ndarray, p_w: np.ndarray, n_c: int, max_row_len: int) -> float:
+    """Bound on sum |p_iI| |p_jJ| over the products of one coarse entry of P^T A P: the largest
+    column sum of |P| times the longest row of A times max |p| (the fixed-point scale of the
+    Galerkin kernels rests on it)."""
+    aw = np.abs(np.asarray(p_w, dtype=np.float64))
+    if aw.size == 0:
+        return 1.0
+    colsum = np.bincount(np.asarray(p_idx, dtype=np.int64), weights=aw, minlength=int(n_c))
+    return float(max(colsum.max(), 1e-300) * max(int(max_row_len), 1) * aw.max())
+
+
 class PMultigridP4:
     """Three-level V-cycle P4 -> P2 -> P1 for order-4 tetrahedra.
 
@@ -409,7 +420,8 @@ class PMultigridP4:
     (Jacobi) smoothing on the P4 and P2 levels, FP32 or FP64."""
 
     def __init__(self, mesh, op, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
-                 coarse_refresh: int = 0, refresh_iterations: int = 20, refresh_ratio: float = 1.5):
+                 coarse_refresh: int = 0, refresh_iterations: int = 20, refresh_ratio: float = 1.5,
+                 coarse: str = "auto"):
         from .device import DevicePattern
         from .fem import geometry, quadrature_3d
         from .pattern import build_owned_pattern
@@ -426,9 +438,12 @@ class PMultigridP4:
         # the P2 level shares the P4 points (same rule), so K_r at that level folds the same
         # per-point shear and lame into its element matrices
         self.A2 = self.pat2.new_csr(with32=True)
+        # the P1 level below: dense, or (large meshes, coarse="auto" above 50k free P1 dofs, or
+        # coarse="aggregate") sparse with the rigid-body-mode aggregate level, which needs the P2
+        # level's node coordinates
         self.coarse = PMultigrid(elem2, q2, use32=use32, degree=degree, lo_frac=lo_frac,
                                  coarse_refresh=coarse_refresh, refresh_iterations=refresh_iterations,
-                                 refresh_ratio=refresh_ratio)
+                                 refresh_ratio=refresh_ratio, coarse=coarse, coord=coord2)
         self.note_iterations = self.coarse.note_iterations
         self.timings = self.coarse.timings
         self.level = None
@@ -576,11 +591,8 @@ class PMultigrid:
     def __init__(self, elem, q_mask, *, use32: bool = True, degree: int = 3, lo_frac: float = 0.1,
                  coarse_refresh: int = 0, coarse_precision: str | None = None, refresh_iterations: int = 20,
                  refresh_ratio: float = 1.5, coarse: str = "auto", coord=None, coarse_dense_max: int = 50000,
-                 degree1: int = 12, p1_refresh: str = "every"):
+                 degree1: int = 12):
         self.use32 = use32
-        if p1_refresh not in ("every", "stale"):
-            raise ValueError(f"p1_refresh must be 'every' or 'stale', not {p1_refresh!r}")
-        self.p1_refresh = p1_refresh
         self.degree = degree
         self.lo_frac = lo_frac
         # coarse_refresh N >= 1: rebuild the coarse operator every N setups; 0: rebuild only when a
@@ -641,6 +653,7 @@ class PMultigrid:
         self.Ainv = None          # packed symmetric tiles of the coarse inverse
         self._tile_part = None
         self._dev_ready = False
+        self._wb = {}
 
     def _to_device(self, dev, n_f):
         p_ptr, p_idx, p_w = self._p_host
@@ -658,6 +671,7 @@ class PMultigrid:
         self.p_idx = torch.from_numpy(p_idx.astype(np.int32)).to(dev)
         self.p_w = torch.from_numpy(p_w).to(dev)
         dt = torch.float32 if self.use32 else torch.float64
+        self._gwork = torch.zeros(1, dtype=torch.int64, device=dev)   # scratch of the Galerkin kernels
         self.rc = torch.empty(self.n_c, dtype=dt, device=dev)
         self.xc = torch.empty(self.n_c, dtype=dt, device=dev)
         if self.coarse == "dense":
@@ -687,6 +701,23 @@ class PMultigrid:
             self.x2 = torch.empty(n_t, dtype=dt, device=dev)
         self._dev_ready = True
 
+    def _w_bound(self, A: DeviceCSR, which: str) -> float:
+        """Weight bound of the fixed-point Galerkin sums for operator A through P ("p") or through
+        the dense level's map ("d"); cached per pattern."""
+        key = (which, int(A.indptr.data_ptr()), A.n_rows)
+        if key not in self._wb:
+            max_len = int((A.indptr[1:] - A.indptr[:-1]).max().item()) if A.n_rows else 0
+            if which == "p":
+                p_ptr, p_idx, p_w = self._p_host
+                n_c = self.n_c
+            elif self.coarse == "dense":
+                p_ptr, p_idx, p_w = self._p_host
+                n_c = self.n_d
+            else:
+                p_ptr, p_idx, p_w, n_c = self._t_host
+            self._wb[key] = galerkin_weight_bound(p_ptr, p_idx, p_w, n_c, max_len)
+        return self._wb[key]
+
     def setup(self, A: DeviceCSR, mesh_info=None):
         dev = A.indptr.device
         if not self._dev_ready:
@@ -699,14 +730,14 @@ class PMultigrid:
         dense_src = A
         due = self.coarse_refresh > 0 and self._setups % self.coarse_refresh == 0
         rebuild = self.Ainv is None or due or self._stale
-        if self.coarse == "aggregate" and (self.p1_refresh == "every" or rebuild):
-            # the sparse P1 level follows the current operator at every setup ("every"), or is
-            # rebuilt with the dense level below it ("stale")
+        if self.coarse == "aggregate":
+            # the sparse P1 level follows the current operator at every setup
             t0 = perf_counter()
             Kc = self.Kc
             Kc.values.zero_()
-            _lib.call("ssx_galerkin_csr", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.p_ptr),
-                      ptr(self.p_idx), ptr(self.p_w), ptr(Kc.indptr), ptr(Kc.indices), ptr(Kc.values), _st())
+            _lib.call("ssx_galerkin_csr", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), A.nnz,
+                      ptr(self.p_ptr), ptr(self.p_idx), ptr(self.p_w), ptr(Kc.indptr), ptr(Kc.indices), Kc.nnz,
+                      self._w_bound(A, "p"), ptr(Kc.values), ptr(self._gwork), _st())
             Kc.values[self.kc_cdiag] = 1.0
             _lib.call("ssx_cast_f64_f32", Kc.nnz, ptr(Kc.values), ptr(Kc.values32), _st())
             if self.level1 is None:
@@ -740,8 +771,9 @@ class PMultigrid:
         t0 = perf_counter()
         self.Ainv = None
         Ac = torch.zeros((self.n_d, self.n_d), dtype=torch.float64, device=dev)
-        _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), ptr(self.d_ptr),
-                  ptr(self.d_idx), ptr(self.d_w), self.n_d, ptr(Ac), _st())
+        _lib.call("ssx_galerkin_dense", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), A.nnz,
+                  ptr(self.d_ptr), ptr(self.d_idx), ptr(self.d_w), self.n_d, self._w_bound(A, "d"), ptr(Ac),
+                  ptr(self._gwork), _st())
         Ac[self.cfix, self.cfix] = 1.0
         torch.cuda.synchronize()
         t1 = perf_counter()

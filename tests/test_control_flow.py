@@ -316,3 +316,44 @@ def test_ll_learned_predictor_slot_gate():
     kinds = {r["predictor"] for r in bad.step_info}
     assert "learned" not in kinds and "learned_gated" in kinds
     assert abs(bad.limit_load - plain.limit_load) < 1e-12
+
+
+def test_reference_probes_take_the_owned_free_path():
+    """The reference's Newton and damping look for the owned/local force builders by name:
+    nonlinear/newton.py:329-338 (callable attribute and a non-None owned_tangent_pattern) and
+    nonlinear/damping.py:105-106,181-182 (callable attribute).  On ConstitutiveOperator those
+    probes succeed, and the builders evaluate in the reference's order (reduction, stress, free
+    rows of the force)."""
+    from slope_stability_petsc4py_amd.constitutive import ConstitutiveOperator
+
+    calls = []
+
+    class Probe(ConstitutiveOperator):
+        def __init__(self):   # no device state: the call order is what is checked
+            self.owned_tangent_pattern = object()
+
+        def reduction(self, lam):
+            calls.append(("reduction", lam))
+
+        def constitutive_problem_stress(self, U):
+            calls.append(("stress", U))
+
+        def build_F_free(self):
+            calls.append(("free",))
+            return "F_free"
+
+    op = Probe()
+
+    def supports_local(builder, name):   # nonlinear/newton.py:334-336
+        return callable(getattr(builder, name, None)) and getattr(builder, "owned_tangent_pattern", None) is not None
+
+    for name in ("build_F_free_local", "build_F_all_free_local", "build_F_reduced_free_local",
+                 "build_F_K_tangent_reduced_free", "build_F_K_regularized_reduced_free"):
+        assert supports_local(op, name), name
+    assert op.build_F_all_free_local(1.3, "U") == "F_free"
+    assert calls == [("reduction", 1.3), ("stress", "U"), ("free",)]
+    calls.clear()
+    assert op.build_F_reduced_free_local("U2") == "F_free"
+    assert calls == [("stress", "U2"), ("free",)]
+    calls.clear()
+    assert op.build_F_free_local() == "F_free" and calls == [("free",)]

@@ -65,8 +65,8 @@ def test_sparse_galerkin_matches_dense_on_the_p1_pattern(gpu, elastic3d):
     from slope_stability_petsc4py_amd.device import ptr, stream_ptr
     n_c = prec.n_c
     Ad = torch.zeros((n_c, n_c), dtype=torch.float64, device=gpu)
-    _lib.call("ssx_galerkin_dense", K.n_rows, ptr(K.indptr), ptr(K.indices), ptr(K.values), ptr(prec.p_ptr),
-              ptr(prec.p_idx), ptr(prec.p_w), n_c, ptr(Ad), stream_ptr())
+    _lib.call("ssx_galerkin_dense", K.n_rows, ptr(K.indptr), ptr(K.indices), ptr(K.values), K.nnz, ptr(prec.p_ptr),
+              ptr(prec.p_idx), ptr(prec.p_w), n_c, prec._w_bound(K, "p"), ptr(Ad), ptr(prec._gwork), stream_ptr())
     D = Ad.cpu().numpy().T          # column-major fill
     fix = np.flatnonzero(~prec.c_free)
     D[fix, fix] = 1.0
@@ -79,6 +79,69 @@ def test_sparse_galerkin_matches_dense_on_the_p1_pattern(gpu, elastic3d):
     assert np.abs(Kc - np.where(pat, D, 0.0)).max() <= 1e-12 * scale
     assert np.abs(D[~pat]).max() <= 1e-12 * scale
     assert mask.sum() > 0 and np.allclose(Kc, Kc.T, rtol=0, atol=1e-12 * scale)
+
+
+@pytest.mark.parametrize("kind", ["dense", "csr"])
+def test_galerkin_bitwise_repeatable_and_matches_scipy(gpu, elastic3d, kind):
+    """The fixed-point Galerkin sums give the same bits on every call, and equal the FP64 product
+    P^T K P of scipy to 1e-13 of its largest entry (the fixed-point resolution is ~1e-16 of it)."""
+    import scipy.sparse as sps
+    from slope_stability_petsc4py_amd import _lib
+    from slope_stability_petsc4py_amd.device import ptr, stream_ptr
+    m, op, K, f = elastic3d
+    prec = PMultigrid(m.elem, m.q_mask, use32=False, coarse="aggregate", coord=m.coord)
+    prec.setup(K)
+    p_ptr, p_idx, p_w = prec._p_host
+    P = sps.csr_matrix((p_w, p_idx, p_ptr), shape=(K.n_rows, prec.n_c))
+    exp = (P.T @ K.to_scipy() @ P).toarray()
+    wb = prec._w_bound(K, "p")
+    runs = []
+    for _ in range(2):
+        if kind == "dense":
+            out = torch.zeros((prec.n_c, prec.n_c), dtype=torch.float64, device=gpu)
+            _lib.call("ssx_galerkin_dense", K.n_rows, ptr(K.indptr), ptr(K.indices), ptr(K.values), K.nnz,
+                      ptr(prec.p_ptr), ptr(prec.p_idx), ptr(prec.p_w), prec.n_c, wb, ptr(out), ptr(prec._gwork),
+                      stream_ptr())
+            got = out.cpu().numpy().T
+        else:
+            out = torch.zeros(prec.Kc.nnz, dtype=torch.float64, device=gpu)
+            _lib.call("ssx_galerkin_csr", K.n_rows, ptr(K.indptr), ptr(K.indices), ptr(K.values), K.nnz,
+                      ptr(prec.p_ptr), ptr(prec.p_idx), ptr(prec.p_w), ptr(prec.Kc.indptr), ptr(prec.Kc.indices),
+                      prec.Kc.nnz, wb, ptr(out), ptr(prec._gwork), stream_ptr())
+            cp = prec.c_pat
+            got = sps.csr_matrix((out.cpu().numpy(), cp.indices, cp.indptr), shape=exp.shape).toarray()
+            on = sps.csr_matrix((np.ones(cp.indices.size), cp.indices, cp.indptr), shape=exp.shape).toarray() != 0
+            exp = np.where(on, exp, 0.0)
+        runs.append(out)
+        scale = np.abs(exp).max()
+        assert np.abs(got - exp).max() <= 1e-13 * scale
+    assert torch.equal(runs[0], runs[1])
+    # through the preconditioner: the sparse P1 level is the same bits on a second setup
+    kc0 = prec.Kc.values.clone()
+    prec.setup(K)
+    assert torch.equal(kc0, prec.Kc.values)
+
+
+@pytest.mark.parametrize("coarse", ["dense", "aggregate"])
+def test_p4_multigrid_coarse_levels_solve(gpu, coarse):
+    """P4 -> P2 -> P1 V-cycle with the dense P1 level and with the sparse P1 level above the
+    aggregate level (which needs the P2 level's coordinates): FGMRES solves the elastic P4 system."""
+    print(f"case: P4 multigrid coarse={coarse}", flush=True)
+    from slope_stability_petsc4py_amd.linear import PMultigridP4
+    m = mesh.slope_3d(h=5.0, nz=1, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, elem_type="P4")
+    op = ConstitutiveOperator.from_mesh(m, [HOMO])
+    op.reduction(1.0)
+    op.constitutive_problem_stress_tangent(np.zeros((3, m.n_nodes)))
+    K = op.build_K_regularized(1e-4)
+    f = op.gravity_load(m).T.reshape(-1).copy()
+    prec = PMultigridP4(m, op, use32=True, coarse=coarse)
+    assert prec.coarse.coarse == coarse
+    solver = DeflatedFGMRES(prec, tolerance=1e-8, max_iterations=300)
+    solver.setup_preconditioner(K)
+    solver.A_orthogonalize(K)
+    x = solver.solve(K, torch.from_numpy(f).to(gpu)).cpu().numpy()
+    Ks = K.to_scipy()
+    assert np.linalg.norm(Ks @ x - f) <= 1.1e-8 * np.linalg.norm(f)
 
 
 def test_deflation_basis_reuses_previous_solutions(gpu, elastic3d):
