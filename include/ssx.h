@@ -136,6 +136,16 @@ int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const
                         const int32_t *indices, const double *val, int lanes, double tol, double *W, double *AW,
                         double *sgn, double *work, void *stream);
 
+/* The same A-orthonormalisation with no scratch argument and no allocation: the scratch is a
+ * file-scope device array of the library (34,304 bytes, one copy per GPU, kept until the process
+ * ends), so kmax is limited to 64 (larger kmax is rejected with status 1).  No counterpart in the
+ * reference: for callers that cannot allocate (code under HIP graph capture, serving loops that
+ * forbid hipMalloc).  Calls that overlap on one GPU race on the shared scratch; the scratch needs
+ * no clearing between calls (every word is written before it is read). */
+int ssx_a_orthogonalize_noalloc(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr,
+                                const int32_t *indices, const double *val, int lanes, double tol, double *W,
+                                double *AW, double *sgn, void *stream);
+
 /* One fused Chebyshev(Jacobi) smoothing step on CSR A: r = b - A x_in (x_in NULL: zero), d = a d +
  * c dinv r (first != 0: d = c dinv r), x_out = x_in + d; x_out must differ from x_in.  Replaces the
  * residual, smoother and update calls of one step of linear/solver.py:155-162,417-436. */

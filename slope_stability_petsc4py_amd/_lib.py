@@ -47,6 +47,7 @@ _SIGS: dict[str, list] = {
     "ssx_sym_tiles_gemv_f32": [I64, P, P, P, P, P],
     "ssx_sym_tiles_gemv_f64": [I64, P, P, P, P, P],
     "ssx_a_orthogonalize": [I64, I32, P, I64, P, P, P, I32, F64, P, P, P, P, P],
+    "ssx_a_orthogonalize_noalloc": [I64, I32, P, I64, P, P, P, I32, F64, P, P, P, P],
     "ssx_cheb_update_f32": [I64, P, P, F32, F32, P, P, P],
     "ssx_cheb_update_f64": [I64, P, P, F64, F64, P, P, P],
     "ssx_inv_diag_f32": [I64, I64, P, P, P, P, P],

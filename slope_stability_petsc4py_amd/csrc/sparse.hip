@@ -10,6 +10,8 @@
 // the Newton operator, FP32 values for the preconditioner; the vector precision matches the values.
 #include "ssx_common.h"
 
+#include <mutex>
+
 namespace {
 
 // Row dot products of CSR rows with x, G lanes per row.
@@ -777,21 +779,24 @@ int ssx_spmm_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, 
     return launch_spmm(n_rows, indptr, indices, val, k, X, ldx, Y, ldy, lanes, ssx::as_stream(stream));
 }
 
-int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr,
-                        const int32_t *indices, const double *val, int lanes, double tol, double *W, double *AW,
-                        double *sgn, double *work, void *stream) {
-    SSX_REQUIRE(n > 0 && kmax >= 0 && ldb >= n, "ssx_a_orthogonalize: bad sizes");
-    SSX_REQUIRE(kmax <= 4096, "ssx_a_orthogonalize: at most 4096 basis columns");
-    if (kmax == 0) return SSX_OK;
-    SSX_REQUIRE(B && indptr && indices && val && W && AW && sgn && work, "ssx_a_orthogonalize: null pointer");
-    hipStream_t st = ssx::as_stream(stream);
+}  // extern "C"
+
+namespace {
+
+// Body of the A-orthonormalisation; `work` holds at least 3*kmax + max(kmax * kDotMaxChunks,
+// kAorthBlocks) doubles.  Every scratch word is written before it is read (h[j] by the
+// coefficient kernel of step j+1, scale/sgn/nv of column i by its finish kernel, the partials by
+// the kernel that produces them), so the scratch needs no clearing.
+int a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr, const int32_t *indices,
+                    const double *val, int lanes, double tol, double *W, double *AW, double *sgn, double *work,
+                    hipStream_t st, const char *name) {
     double *h = work;                 // kmax
     double *scale = work + kmax;      // kmax
     double *nv = work + 2 * kmax;     // kmax
     double *part = work + 3 * kmax;   // max(kmax * kDotMaxChunks, kAorthBlocks)
     const int64_t total = n * (int64_t)kmax;
     hipLaunchKernelGGL(k_copy_rows_rev, dim3(ssx::grid_for(total, 256)), dim3(256), 0, st, n, kmax, B, ldb, W);
-    SSX_CHECK_LAUNCH("ssx_a_orthogonalize");
+    SSX_CHECK_LAUNCH(name);
     const int rc = launch_spmm(n, indptr, indices, val, kmax, W, n, AW, n, lanes, st);
     if (rc != SSX_OK) return rc;
     int64_t n_chunk = (n + kDotChunk - 1) / kDotChunk;
@@ -809,8 +814,77 @@ int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const
         hipLaunchKernelGGL(k_aorth_finish, dim3(1), dim3(256), 0, st, nb, part, i, tol, nv, sgn, scale);
     }
     hipLaunchKernelGGL(k_aorth_scale, dim3(ssx::grid_for(total, 256)), dim3(256), 0, st, n, kmax, scale, W, AW);
-    SSX_CHECK_LAUNCH("ssx_a_orthogonalize");
+    SSX_CHECK_LAUNCH(name);
     return SSX_OK;
+}
+
+// Scratch of the allocation-free A-orthonormalisation: one copy per GPU, held until the process
+// ends.  Its device address comes from hipGetSymbolAddress, once per device, with that device
+// current; host code uses only that address (the array's name is a host-side stand-in).
+constexpr int kStaticKmax = 64;
+constexpr int64_t kStaticWork =
+    3 * kStaticKmax + (kStaticKmax * kDotMaxChunks > kAorthBlocks ? kStaticKmax * kDotMaxChunks : kAorthBlocks);
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+static __device__ __align__(256) double scratch[kStaticWork];
+
+namespace {
+// Device address of `scratch` on the current device (looked up on the first call per device).
+int scratch_address(double **out, const char *name) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        ssx::set_error("%s: hipGetDevice failed", name);
+        return SSX_ERUNTIME;
+    }
+    if (dev < 0 || dev >= kMaxDevices) {
+        ssx::set_error("%s: device %d beyond the %d devices the scratch table holds", name, dev, kMaxDevices);
+        return SSX_ERUNTIME;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[dev] == nullptr) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || p == nullptr) {
+            ssx::set_error("%s: hipGetSymbolAddress(scratch) failed: %s", name, hipGetErrorString(e));
+            return SSX_ERUNTIME;
+        }
+        g_scratch_ptr[dev] = static_cast<double *>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return SSX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ssx_a_orthogonalize(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr,
+                        const int32_t *indices, const double *val, int lanes, double tol, double *W, double *AW,
+                        double *sgn, double *work, void *stream) {
+    SSX_REQUIRE(n > 0 && kmax >= 0 && ldb >= n, "ssx_a_orthogonalize: bad sizes");
+    SSX_REQUIRE(kmax <= 4096, "ssx_a_orthogonalize: at most 4096 basis columns");
+    if (kmax == 0) return SSX_OK;
+    SSX_REQUIRE(B && indptr && indices && val && W && AW && sgn && work, "ssx_a_orthogonalize: null pointer");
+    return a_orthogonalize(n, kmax, B, ldb, indptr, indices, val, lanes, tol, W, AW, sgn, work,
+                           ssx::as_stream(stream), "ssx_a_orthogonalize");
+}
+
+int ssx_a_orthogonalize_noalloc(int64_t n, int kmax, const double *B, int64_t ldb, const int32_t *indptr,
+                                const int32_t *indices, const double *val, int lanes, double tol, double *W,
+                                double *AW, double *sgn, void *stream) {
+    SSX_REQUIRE(n > 0 && kmax >= 0 && ldb >= n, "ssx_a_orthogonalize_noalloc: bad sizes");
+    SSX_REQUIRE(kmax <= kStaticKmax, "ssx_a_orthogonalize_noalloc: at most %d basis columns (the static scratch)",
+                kStaticKmax);
+    if (kmax == 0) return SSX_OK;
+    SSX_REQUIRE(B && indptr && indices && val && W && AW && sgn, "ssx_a_orthogonalize_noalloc: null pointer");
+    double *work = nullptr;
+    const int rc = scratch_address(&work, "ssx_a_orthogonalize_noalloc");
+    if (rc != SSX_OK) return rc;
+    return a_orthogonalize(n, kmax, B, ldb, indptr, indices, val, lanes, tol, W, AW, sgn, work,
+                           ssx::as_stream(stream), "ssx_a_orthogonalize_noalloc");
 }
 
 }  // extern "C"
