@@ -409,9 +409,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--no-ssr", action="store_true", help="skip the end-to-end SSR leg")
-    ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows", "elem"],
-                    help="P2 tangent path: element matrices + node gather (default), fused row kernel, or "
-                         "element blocks + row gather")
+    ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows"],
+                    help="P2 tangent path: element matrices + node gather (default) or the fused row kernel")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the path runs on a single GPU, and sharding it over several "

@@ -94,11 +94,9 @@ int ssx_tangent_rows(int dim, int n_p, const double *dphi, const double *DS, con
                      int64_t n_elem, int n_q, double r, const double *elastic, double *out, float *out32,
                      int64_t max_block_nnz, void *stream);
 
-/* SpMV row kernel (tuning knob, also used by the fused Chebyshev step): 0 = one value and one
- * index per lane and load, 1 = aligned quadruples per lane (16-byte index and value loads) where
- * the arrays are 16-byte aligned and a row has at least 4 lanes.  No counterpart in the reference. */
-int ssx_set_spmv_kernel(int vec4);
-/* y = alpha*A x + beta*y for CSR A (int32 indptr/indices); lanes per row: power of two <= 64.
+/* y = alpha*A x + beta*y for CSR A (int32 indptr/indices); lanes per row: power of two <= 64.  With
+ * at least 4 lanes and 16-byte aligned indices/val, each lane reads aligned quadruples of entries
+ * (the same row form serves the fused Chebyshev step); otherwise one entry per lane and load.
  * Replaces PETSc MatMult (linear/solver.py:3431-3451). */
 int ssx_spmv_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, const double *x,
                  double *y, double alpha, double beta, int lanes, void *stream);
@@ -176,16 +174,12 @@ int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *w, int64_t n_elem, double r,
                             const double *shear, const double *lame, double *ke, void *stream);
 
-/* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p, row-major), one
- * wavefront per element; optional K_r fold exactly as ssx_element_matrices_p4.  Replaces the
+/* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p, row-major):
+ * persistent wavefronts with the next element's point data in flight (11-point P2 and 1-point P1
+ * rules), one wavefront per element otherwise; optional K_r fold exactly as ssx_element_matrices_p4.  Replaces the
  * element stage of cython/assemble_tangent_values_3d.c:13-109. */
 int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
                             double r, const double *shear, const double *lame, double *ke, void *stream);
-
-/* Tuning knob of ssx_element_matrices_3d (no reference counterpart): variant 0 = one wavefront per
- * element with direct loads, 1 = persistent pipelined wavefronts with D from LDS (default),
- * 2 = pipelined with D by scalar loads; waves_per_cu sizes the persistent grid. */
-int ssx_set_element_kernel(int variant, int waves_per_cu);
 
 /* Node gather of element matrices into the fixed CSR pattern: one wavefront per mesh node sums the
  * node's dim rows over its elements (node slots ns_ptr/ns_elem/ns_lnode, element order) at the
@@ -199,21 +193,6 @@ int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_p
                          const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
                          const uint8_t *row_free, int64_t n_nodes, int max_len, double r, const double *elastic,
                          double *out, float *out32, void *stream);
-
-/* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p) by 3x3 node
- * blocks on the vector ALUs (one lane per node pair i <= j, mirrored).  Same reference stage as
- * ssx_element_matrices_p4 (cython/assemble_tangent_values_3d.c:45-94). */
-int ssx_element_matrices_blocks(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem,
-                                int n_q, double *ke, void *stream);
-
-/* Row gather of element matrices into the fixed CSR pattern through the row-slot lists (slot order
- * = element order), with unit constrained diagonals, optional K_r blend and FP32 copy.  Together
- * with ssx_element_matrices_p4 replaces assemble_tangent_values_3d_rows_c for P4
- * (cython/assemble_tangent_values_3d.c:111-214). */
-int ssx_scatter_element_rows(const double *ke, int nld, const int32_t *row_slot_ptr, const int32_t *slot_elem,
-                             const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr,
-                             const uint8_t *row_free, int64_t n_rows, double r, const double *elastic, double *out,
-                             float *out32, int64_t max_block_nnz, void *stream);
 
 /* ---- learned continuation predictor (no reference counterpart: it fills the predictor slot of
  * continuation/indirect.py:2703-2830, where the reference uses the secant of :550-566) ---- */

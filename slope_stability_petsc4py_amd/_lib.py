@@ -30,7 +30,6 @@ _SIGS: dict[str, list] = {
     "ssx_force_gather_nodes": [I32, I32, P, P, P, P, P, I64, P, P],
     "ssx_tangent_rows": [I32, I32, P, P, P, P, P, P, P, P, P, I64, I64, I32, F64, P, P, P, I64, P],
     "ssx_spmv_f64": [I64, P, P, P, P, P, F64, F64, I32, P],
-    "ssx_set_spmv_kernel": [I32],
     "ssx_spmv_f32": [I64, P, P, P, P, P, F32, F32, I32, P],
     "ssx_axpby_f64": [I64, F64, P, F64, P, P],
     "ssx_axpby_f32": [I64, F32, P, F32, P, P],
@@ -56,9 +55,7 @@ _SIGS: dict[str, list] = {
     "ssx_galerkin_csr": [I64, P, P, P, I64, P, P, P, P, P, I64, F64, P, P, P],
     "ssx_element_matrices_p4": [P, P, P, I64, F64, P, P, P, P],
     "ssx_element_matrices_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P],
-    "ssx_set_element_kernel": [I32, I32],
     "ssx_gather_node_rows": [I32, I32, P, P, P, P, P, P, P, I64, I32, F64, P, P, P, P],
-    "ssx_scatter_element_rows": [P, I32, P, P, P, P, P, P, I64, F64, P, P, P, I64, P],
     "ssx_gemm_bf16": [I64, I64, I64, P, I64, I64, P, I64, I64, P, I64, P, I32, P, F32, P],
     "ssx_tanh_bwd": [I64, P, P, P, P, P],
     "ssx_colsum_f32": [I64, I64, P, P, P],
@@ -66,7 +63,6 @@ _SIGS: dict[str, list] = {
     "ssx_f32_to_bf16": [I64, P, P, P],
     "ssx_bf16_to_f32": [I64, P, P, P],
     "ssx_mse_f32": [I64, P, P, P, F32, P, P, P],
-    "ssx_element_matrices_blocks": [I32, P, P, P, I64, I32, P, P],
     "ssx_stream_sync": [P],
 }
 

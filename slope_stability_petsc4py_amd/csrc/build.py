@@ -26,7 +26,6 @@ SOURCES = [
     "coarse.hip",
     "p4_mfma.hip",
     "mlp.hip",
-    "elem_blocks.hip",
     "node_assembly.hip",
 ]
 

@@ -124,8 +124,6 @@ __global__ void __launch_bounds__(64) k_ke_3d(const double *__restrict__ dphi, c
 // LDS per buffer: gradients sg[q][c][h][6] (the two column halves of NH <= 5 nodes each padded to
 // 6, so a lane reads its columns as aligned pairs), D transposed to rows sd[q][s][m] = D[s + 6 m]
 // (a lane reads the three rows of D its B row touches), and w, shear, lame per point.
-// DSMEM: D is read with scalar loads from global memory instead (the staging of D into LDS then
-// only brings the element's D into the L2 before those scalar loads).
 // ------------------------------------------------------------------------------------------
 template <int NP, int NQ>
 struct KePipe {
@@ -141,7 +139,7 @@ struct KePipe {
     static constexpr int NMR = (M + 63) / 64;
 };
 
-template <int NP, int NQ, bool FOLD, bool DSMEM, int NBUF>
+template <int NP, int NQ, bool FOLD>
 __device__ __forceinline__ void ke_pipe_body(const double *__restrict__ dphi, const double *__restrict__ DS,
                                              const double *__restrict__ w, int64_t n_elem, double r,
                                              const double *__restrict__ shear, const double *__restrict__ lame,
@@ -248,28 +246,11 @@ __device__ __forceinline__ void ke_pipe_body(const double *__restrict__ dphi, co
             const double wo = wg * om;
             const double wc0 = wo * c0, wc1 = wo * c1, wc2 = wo * c2;
             double t[6];
-            if constexpr (DSMEM) {
-                const double *Dg = DS + (e * NQ + q) * 36;   // column-major D[s + 6 m]
+            const double *D0 = L + P::G + q * 36 + s0 * 6;
+            const double *D1 = L + P::G + q * 36 + s1 * 6;
+            const double *D2 = L + P::G + q * 36 + s2 * 6;
 #pragma unroll
-                for (int m = 0; m < 6; ++m) {
-                    // rows s0, s1, s2 of column m, selected by value from the six entries
-                    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-#pragma unroll
-                    for (int ss = 0; ss < 6; ++ss) {
-                        const double v = Dg[ss + 6 * m];
-                        d0 = ss == s0 ? v : d0;
-                        d1 = ss == s1 ? v : d1;
-                        d2 = ss == s2 ? v : d2;
-                    }
-                    t[m] = fma(wc2, d2, fma(wc1, d1, wc0 * d0));
-                }
-            } else {
-                const double *D0 = L + P::G + q * 36 + s0 * 6;
-                const double *D1 = L + P::G + q * 36 + s1 * 6;
-                const double *D2 = L + P::G + q * 36 + s2 * 6;
-#pragma unroll
-                for (int m = 0; m < 6; ++m) t[m] = fma(wc2, D2[m], fma(wc1, D1[m], wc0 * D0[m]));
-            }
+            for (int m = 0; m < 6; ++m) t[m] = fma(wc2, D2[m], fma(wc1, D1[m], wc0 * D0[m]));
             if constexpr (FOLD) {
                 // + w r D_el b_a with D_el b = lame (b0 + b1 + b2) vol + 2 G ident b; here
                 // b0 + b1 + b2 = c0 and b is (c0, c1, c2) on rows (s0, s1, s2)
@@ -301,36 +282,20 @@ __device__ __forceinline__ void ke_pipe_body(const double *__restrict__ dphi, co
 #pragma unroll
             for (int k = 0; k < 3 * NH; ++k) o[k] = acc[k];
         }
-        if constexpr (NBUF == 2) {
-            if (en < n_elem) stage(buf ^ 1);
-            __syncthreads();
-            buf ^= 1;
-        } else {
-            __syncthreads();   // every lane is done reading the buffer
-            if (en < n_elem) stage(0);
-            __syncthreads();
-        }
+        if (en < n_elem) stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
 // Double-buffered LDS (the next element is staged while nothing waits for it).
-template <int NP, int NQ, bool FOLD, bool DSMEM>
+template <int NP, int NQ, bool FOLD>
 __global__ void __launch_bounds__(64) k_ke_3d_pipe(const double *__restrict__ dphi, const double *__restrict__ DS,
                                                    const double *__restrict__ w, int64_t n_elem, double r,
                                                    const double *__restrict__ shear, const double *__restrict__ lame,
                                                    double *__restrict__ ke) {
     __shared__ double lds[2][KePipe<NP, NQ>::WORDS];
-    ke_pipe_body<NP, NQ, FOLD, DSMEM, 2>(dphi, DS, w, n_elem, r, shear, lame, ke, lds);
-}
-
-// Single LDS buffer and at least four wavefronts per SIMD (register budget 128).
-template <int NP, int NQ, bool FOLD, bool DSMEM>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4)))
-k_ke_3d_pipe1(const double *__restrict__ dphi, const double *__restrict__ DS, const double *__restrict__ w,
-              int64_t n_elem, double r, const double *__restrict__ shear, const double *__restrict__ lame,
-              double *__restrict__ ke) {
-    __shared__ double lds[1][KePipe<NP, NQ>::WORDS];
-    ke_pipe_body<NP, NQ, FOLD, DSMEM, 1>(dphi, DS, w, n_elem, r, shear, lame, ke, lds);
+    ke_pipe_body<NP, NQ, FOLD>(dphi, DS, w, n_elem, r, shear, lame, ke, lds);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -465,21 +430,11 @@ int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_ele
 
 }  // namespace
 
-// Element-matrix kernel selection (tuning knob): 0 = one wavefront per element with direct
-// loads, 1 = pipelined with D from LDS, 2 = pipelined with D by scalar loads, 3 = pipelined with a
-// single LDS buffer and a 128-register budget; waves per CU of the persistent grid.
-static int ssx_ke_variant = 1;
-static int ssx_ke_waves_per_cu = 24;
+// Persistent grid of the pipelined element-matrix kernel: 24 wavefronts per CU (measured best
+// of 8-32 on MI355X; the kernel holds 8 resident per CU).
+constexpr int kKeWavesPerCU = 24;
 
 extern "C" {
-
-int ssx_set_element_kernel(int variant, int waves_per_cu) {
-    SSX_REQUIRE(variant >= 0 && variant <= 3 && waves_per_cu > 0 && waves_per_cu <= 64,
-                "ssx_set_element_kernel: variant 0..3, waves_per_cu 1..64");
-    ssx_ke_variant = variant;
-    ssx_ke_waves_per_cu = waves_per_cu;
-    return SSX_OK;
-}
 
 int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
                             double r, const double *shear, const double *lame, double *ke, void *stream) {
@@ -489,8 +444,7 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
     SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_matrices_3d: shear and lame go together");
     SSX_REQUIRE(n_elem <= 0x7fffffff, "ssx_element_matrices_3d: too many elements for one launch");
     hipStream_t st = ssx::as_stream(stream);
-    const int variant = ssx_ke_variant;
-    if (variant > 0 && ((n_p == 10 && n_q == 11) || (n_p == 4 && n_q == 1))) {
+    if ((n_p == 10 && n_q == 11) || (n_p == 4 && n_q == 1)) {
         static int n_cu = 0;
         if (n_cu == 0) {
             int dev = 0;
@@ -498,19 +452,15 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
                 hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
                 n_cu = 256;
         }
-        const int64_t want = (int64_t)n_cu * ssx_ke_waves_per_cu;
+        const int64_t want = (int64_t)n_cu * kKeWavesPerCU;
         const unsigned grid = (unsigned)(n_elem < want ? n_elem : want);
         const bool fold = shear != nullptr;
-#define PIPE(K_, NP_, NQ_, F_, S_) \
-    hipLaunchKernelGGL((K_<NP_, NQ_, F_, S_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, ke)
+#define PIPE(NP_, NQ_, F_) \
+    hipLaunchKernelGGL((k_ke_3d_pipe<NP_, NQ_, F_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, ke)
         if (n_p == 10) {
-            if (variant == 1) { if (fold) PIPE(k_ke_3d_pipe, 10, 11, true, false); else PIPE(k_ke_3d_pipe, 10, 11, false, false); }
-            else if (variant == 2) { if (fold) PIPE(k_ke_3d_pipe, 10, 11, true, true); else PIPE(k_ke_3d_pipe, 10, 11, false, true); }
-            else { if (fold) PIPE(k_ke_3d_pipe1, 10, 11, true, false); else PIPE(k_ke_3d_pipe1, 10, 11, false, false); }
+            if (fold) PIPE(10, 11, true); else PIPE(10, 11, false);
         } else {
-            if (variant == 1) { if (fold) PIPE(k_ke_3d_pipe, 4, 1, true, false); else PIPE(k_ke_3d_pipe, 4, 1, false, false); }
-            else if (variant == 2) { if (fold) PIPE(k_ke_3d_pipe, 4, 1, true, true); else PIPE(k_ke_3d_pipe, 4, 1, false, true); }
-            else { if (fold) PIPE(k_ke_3d_pipe1, 4, 1, true, false); else PIPE(k_ke_3d_pipe1, 4, 1, false, false); }
+            if (fold) PIPE(4, 1, true); else PIPE(4, 1, false);
         }
 #undef PIPE
         SSX_CHECK_LAUNCH("ssx_element_matrices_3d");

@@ -1,5 +1,4 @@
-// P4 tetrahedra: element tangent matrices on the FP64 matrix cores, then a deterministic row
-// gather into the fixed CSR pattern.
+// P4 tetrahedra: element tangent matrices on the FP64 matrix cores.
 //
 // ke = sum_q B_q^T (w_q D_q) B_q is one GEMM per element, ke (105 x 105) = A (105 x 144) * Bm (144 x 105)
 // with K = 24 points x 6 strains, A[i][k] = B_q[s][i] and Bm[k][j] = sum_t w_q D_q[s][t] B_q[t][j]
@@ -9,12 +8,12 @@
 // so an operand value costs <= 3 FMAs).  DS is symmetric for associated flow, so only the upper
 // tile triangle (28 of 49 tiles) is computed and mirrored on write.
 //
-// Stage 2 walks every row's slots in element order (the reference's `rows` order,
-// cython/assemble_tangent_values_3d.c:111-214) and adds ke[e][a][:] at the slot's CSR positions,
-// then applies the constrained diagonal, K_r blend and FP32 copy.
+// The element matrices are then gathered per mesh node into the fixed CSR pattern
+// (ssx_gather_node_rows in node_assembly.hip), in the reference's `rows` summation order
+// (cython/assemble_tangent_values_3d.c:111-214).
 //
-// Reference behaviour: cython/assemble_tangent_values_3d.c:13-109 (element matrices + scatter),
-// :111-214 (rows); flop count 2*105*105*144 per element.
+// Reference behaviour: cython/assemble_tangent_values_3d.c:13-109 (element matrices); flop count
+// 2*105*105*144 per element.
 #include "ssx_common.h"
 
 namespace {
@@ -137,108 +136,6 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
     }
 }
 
-// Row gather of element matrices: one lane per row, slots in element order; LDS row-block tile
-// as in the P2 tangent kernel (USE_LDS) or read-modify-write on the lane's own row in HBM.
-// NLD > 0: compile-time element size, columns handled in register chunks of CH (positions and
-// values loaded as vectors, then branch-free LDS updates with constrained columns sent to the
-// dummy word buf[L]); NLD == 0: any element size.
-constexpr int kRowsKe = 64;
-
-template <int NLD, bool USE_LDS>
-__global__ void __launch_bounds__(kRowsKe) k_scatter_ke_rows(const double *__restrict__ ke, int nld,
-                                                             const int32_t *__restrict__ row_slot_ptr,
-                                                             const int32_t *__restrict__ slot_elem,
-                                                             const uint8_t *__restrict__ slot_lrow,
-                                                             const int32_t *__restrict__ slot_pos,
-                                                             const int32_t *__restrict__ indptr,
-                                                             const uint8_t *__restrict__ row_free, int64_t n_rows,
-                                                             double r, const double *__restrict__ elastic,
-                                                             double *__restrict__ out, float *__restrict__ out32) {
-    extern __shared__ double lds[];
-    const int64_t r0 = (int64_t)blockIdx.x * kRowsKe;
-    const int64_t r1 = r0 + kRowsKe < n_rows ? r0 + kRowsKe : n_rows;
-    const int64_t v0 = indptr[r0];
-    const int64_t L = (int64_t)indptr[r1] - v0;
-    double *buf = USE_LDS ? lds : out + v0;
-    for (int64_t i = threadIdx.x; i < L; i += kRowsKe) buf[i] = 0.0;
-    __syncthreads();
-    const int64_t row = r0 + threadIdx.x;
-    if (row < r1) {
-        if (!row_free[row]) {
-            buf[indptr[row] - v0] = 1.0;
-        } else {
-            const int32_t s1 = row_slot_ptr[row + 1];
-            for (int32_t s = row_slot_ptr[row]; s < s1; ++s) {
-                if constexpr (NLD > 0 && USE_LDS) {
-                    constexpr int CH = NLD % 15 == 0 ? 15 : NLD;
-                    const double *kr = ke + ((int64_t)slot_elem[s] * NLD + slot_lrow[s]) * NLD;
-                    const int32_t *pos = slot_pos + (int64_t)s * NLD;
-                    for (int b0 = 0; b0 < NLD; b0 += CH) {
-                        int32_t pv[CH];
-                        double kv[CH];
-#pragma unroll
-                        for (int b = 0; b < CH; ++b) {
-                            pv[b] = pos[b0 + b];
-                            kv[b] = kr[b0 + b];
-                        }
-#pragma unroll
-                        for (int b = 0; b < CH; ++b) {
-                            const int64_t idx = pv[b] >= 0 ? (int64_t)pv[b] - v0 : L;
-                            buf[idx] += kv[b];
-                        }
-                    }
-                } else {
-                    const double *kr = ke + ((int64_t)slot_elem[s] * nld + slot_lrow[s]) * nld;
-                    const int32_t *pos = slot_pos + (int64_t)s * nld;
-                    for (int b = 0; b < nld; ++b) {
-                        const int32_t p = pos[b];
-                        if (p >= 0) buf[p - v0] += kr[b];
-                    }
-                }
-            }
-        }
-    }
-    __syncthreads();
-    const double om = 1.0 - r;
-    for (int64_t i = threadIdx.x; i < L; i += kRowsKe) {
-        double v = buf[i];
-        if (elastic) v = om * v + r * elastic[v0 + i];
-        if (USE_LDS) out[v0 + i] = v;
-        else buf[i] = v;
-        if (out32) out32[v0 + i] = (float)v;
-    }
-}
-
-template <int NLD>
-int launch_scatter(const double *ke, int nld, const int32_t *row_slot_ptr, const int32_t *slot_elem,
-                   const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr, const uint8_t *row_free,
-                   int64_t n_rows, double r, const double *elastic, double *out, float *out32, int64_t max_block_nnz,
-                   hipStream_t st) {
-    const unsigned grid = ssx::grid_for(n_rows, kRowsKe);
-    const size_t lds_bytes = ((size_t)max_block_nnz + 1) * sizeof(double);  // + the dummy word
-    static int max_lds = -1;
-    if (max_lds < 0) {
-        int dev = 0, v = 65536;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            v = 65536;
-        max_lds = v;
-    }
-    bool lds_ok = max_block_nnz >= 0 && lds_bytes <= (size_t)max_lds;
-    if (lds_ok && lds_bytes > 65536)
-        lds_ok = hipFuncSetAttribute(reinterpret_cast<const void *>(&k_scatter_ke_rows<NLD, true>),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) == hipSuccess;
-    if (lds_ok)
-        hipLaunchKernelGGL((k_scatter_ke_rows<NLD, true>), dim3(grid), dim3(kRowsKe), lds_bytes, st, ke, nld,
-                           row_slot_ptr, slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out,
-                           out32);
-    else
-        hipLaunchKernelGGL((k_scatter_ke_rows<NLD, false>), dim3(grid), dim3(kRowsKe), 0, st, ke, nld, row_slot_ptr,
-                           slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, elastic, out, out32);
-    SSX_CHECK_LAUNCH("ssx_scatter_element_rows");
-    return SSX_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -253,23 +150,6 @@ int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *
                        n_elem * NQ4, r, shear, lame, ke);
     SSX_CHECK_LAUNCH("ssx_element_matrices_p4");
     return SSX_OK;
-}
-
-int ssx_scatter_element_rows(const double *ke, int nld, const int32_t *row_slot_ptr, const int32_t *slot_elem,
-                             const uint8_t *slot_lrow, const int32_t *slot_pos, const int32_t *indptr,
-                             const uint8_t *row_free, int64_t n_rows, double r, const double *elastic, double *out,
-                             float *out32, int64_t max_block_nnz, void *stream) {
-    SSX_REQUIRE(n_rows >= 0 && nld > 0, "ssx_scatter_element_rows: bad sizes");
-    if (n_rows == 0) return SSX_OK;
-    SSX_REQUIRE(ke && row_slot_ptr && indptr && row_free && out, "ssx_scatter_element_rows: null pointer");
-    hipStream_t st = ssx::as_stream(stream);
-#define CALL(N) launch_scatter<N>(ke, nld, row_slot_ptr, slot_elem, slot_lrow, slot_pos, indptr, row_free, n_rows, r, \
-                                  elastic, out, out32, max_block_nnz, st)
-    if (nld == 12) return CALL(12);
-    if (nld == 30) return CALL(30);
-    if (nld == 105) return CALL(105);
-    return CALL(0);
-#undef CALL
 }
 
 }  // extern "C"

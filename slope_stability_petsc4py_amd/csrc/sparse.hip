@@ -64,12 +64,11 @@ __device__ __forceinline__ T row_dot(int32_t k0, int32_t k1, int lane, const int
     return acc;
 }
 
-// SpMV kernel selection (tuning knob): 0 = scalar rows, 1 = vec4 rows where the arrays are aligned
-// and the row group has at least 4 lanes (default: 13-18% faster on the 3D P2/P4 operators).
-static int ssx_spmv_vec4 = 1;
-
+// Row form: aligned quadruples (16-byte index and value loads) where the index and value arrays are
+// 16-byte aligned and a row has at least 4 lanes (13-18% faster than one entry per lane on the 3D
+// P2/P4 operators); one entry per lane otherwise.
 __host__ inline bool vec4_ok(const void *indices, const void *val, int lanes) {
-    return ssx_spmv_vec4 && lanes >= 4 && ((uintptr_t)indices % 16 == 0) && ((uintptr_t)val % 16 == 0);
+    return lanes >= 4 && ((uintptr_t)indices % 16 == 0) && ((uintptr_t)val % 16 == 0);
 }
 
 template <typename T, int G, bool V4>
@@ -299,12 +298,6 @@ __global__ void __launch_bounds__(256) k_inv_diag(int64_t n_rows, int64_t col0, 
 }  // namespace
 
 extern "C" {
-
-int ssx_set_spmv_kernel(int vec4) {
-    SSX_REQUIRE(vec4 == 0 || vec4 == 1, "ssx_set_spmv_kernel: 0 (scalar rows) or 1 (vec4 rows)");
-    ssx_spmv_vec4 = vec4;
-    return SSX_OK;
-}
 
 int ssx_spmv_f64(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, const double *x,
                  double *y, double alpha, double beta, int lanes, void *stream) {

@@ -187,21 +187,14 @@ class DevicePattern:
                   self.n_q, float(r), ptr(sh), ptr(la), ptr(ke), stream_ptr())
         return ke
 
-    def element_matrices_blocks(self, DS: torch.Tensor) -> torch.Tensor:
-        """(n_elem, nld, nld) element tangents of a P1/P2 tetrahedral mesh (node-block kernel)."""
-        ke = self._ke_buffer(DS.device)
-        _lib.call("ssx_element_matrices_blocks", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
-                  self.n_q, ptr(ke), stream_ptr())
-        return ke
-
     def gather_nodes(self, ke: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
                      elastic: torch.Tensor | None = None) -> None:
         _lib.call("ssx_gather_node_rows", self.dim, self.nld, ptr(ke), ptr(self.ns_ptr), ptr(self.ns_elem),
                   ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), self.host.n_nodes,
                   self.ns.max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
 
-    # 3D tetrahedra: "nodes" (element matrices + node gather, the default), "rows" (fused row
-    # kernel) or "elem" (node-block element matrices + row gather)
+    # 3D tetrahedra: "nodes" (element matrices + node gather, the default) or "rows" (fused row
+    # kernel, the 2D path)
     tangent_method = "nodes"
 
     def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
@@ -219,12 +212,6 @@ class DevicePattern:
             return
         if mat is not None:
             raise ValueError("the K_r fold needs the node-gather path")
-        if self.dim == 3 and self.tangent_method == "elem":
-            ke = self.element_matrices_blocks(DS)
-            _lib.call("ssx_scatter_element_rows", ptr(ke), self.nld, ptr(self.row_slot_ptr), ptr(self.slot_elem),
-                      ptr(self.slot_lrow), ptr(self.slot_pos), ptr(self.indptr), ptr(self.row_free), self.n_rows,
-                      float(r), ptr(elastic), ptr(out), ptr(out32), self.max_block_nnz, stream_ptr())
-            return
         _lib.call("ssx_tangent_rows", self.dim, self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight),
                   ptr(self.row_slot_ptr), ptr(self.slot_elem), ptr(self.slot_lrow), ptr(self.slot_pos),
                   ptr(self.indptr), ptr(self.row_free), self.n_rows, self.n_elem, self.n_q, float(r),

@@ -208,17 +208,16 @@ def test_spmv_vs_oracle(gpu, asm3d):
     assert _rel(y.cpu().numpy(), yo) < 1e-13
     y32 = K.matvec32(torch.from_numpy(x.astype(np.float32)).to(dev))
     assert _rel(y32.cpu().numpy().astype(np.float64), yo) < 1e-5
-    # both row kernels: scalar rows, and aligned quadruples with head/tail entries by lanes 0..2
-    for vec4 in (0, 1):
-        _lib.call("ssx_set_spmv_kernel", vec4)
-        try:
-            for lanes in (1, 2, 4, 8, 16, 32, 64):
-                K.lanes = lanes
-                assert _rel(K.matvec(torch.from_numpy(x).to(dev)).cpu().numpy(), yo) < 1e-13
-                y32 = K.matvec32(torch.from_numpy(x.astype(np.float32)).to(dev))
-                assert _rel(y32.cpu().numpy().astype(np.float64), yo) < 1e-5
-        finally:
-            _lib.call("ssx_set_spmv_kernel", 1)
+    # both row forms: aligned quadruples with head/tail entries by lanes 0..2 (16-byte aligned
+    # arrays, >= 4 lanes), and one entry per lane (fewer lanes, or arrays off the 16-byte grid)
+    from gpu_helpers import misaligned
+    Km = misaligned(K)
+    for KK in (K, Km):
+        for lanes in (1, 2, 4, 8, 16, 32, 64):
+            KK.lanes = lanes
+            assert _rel(KK.matvec(torch.from_numpy(x).to(dev)).cpu().numpy(), yo) < 1e-13
+            y32 = KK.matvec32(torch.from_numpy(x.astype(np.float32)).to(dev))
+            assert _rel(y32.cpu().numpy().astype(np.float64), yo) < 1e-5
 
 
 @pytest.mark.parametrize("k", [1, 8, 13])
@@ -358,8 +357,9 @@ def test_p4_element_matrices_mfma_vs_oracle(gpu):
     assert _rel(K.cpu().numpy(), Ko) < 1e-12
 
 
-def test_p2_element_blocks_path_vs_oracle(gpu, asm3d):
-    """Element-block matrices + row gather (the alternative P2 tangent path) == oracle rows kernel."""
+def test_p2_node_gather_elastic_blend_vs_oracle(gpu, asm3d):
+    """Element matrices + node gather with the assembled-elastic K_r blend (1-r) K + r K_el and the
+    FP32 copy == the oracle rows kernel blended the same way (constrained rows blended too)."""
     m, geom, pat = asm3d
     dp = DevicePattern(pat, geom)
     dev = dp.indptr.device
@@ -367,18 +367,19 @@ def test_p2_element_blocks_path_vs_oracle(gpu, asm3d):
     A = rng.normal(size=(geom.n_int, 6, 6))
     DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
     DSd = torch.from_numpy(DS).to(dev)
-    ke = dp.element_matrices_blocks(DSd).cpu().numpy()
+    ke = dp.element_matrices_3d(DSd).cpu().numpy()
     keo = oracle.element_matrices_3d(geom.dphi, DS, geom.weight, geom.n_elem, geom.n_q)
     assert _rel(ke, keo) < 1e-12
-    dp.tangent_method = "elem"
     K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
     K32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev)
-    el = torch.from_numpy(rng.normal(size=pat.nnz)).to(dev)
-    dp.tangent(DSd, K, K32, r=0.2, elastic=el)
+    el = rng.normal(size=pat.nnz)
+    el[pat.constrained_diag_positions] = 1.0    # K_el, like K, has unit constrained rows
+    dp.tangent(DSd, K, K32, r=0.2, elastic=torch.from_numpy(el).to(dev))
     Ko = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
-    exp = 0.8 * Ko + 0.2 * el.cpu().numpy()
-    exp[pat.constrained_diag_positions] = 0.8 + 0.2 * el.cpu().numpy()[pat.constrained_diag_positions]
+    exp = 0.8 * Ko + 0.2 * el
     assert _rel(K.cpu().numpy(), exp) < 1e-12
+    assert np.all(K.cpu().numpy()[pat.constrained_diag_positions] == 1.0)
+    np.testing.assert_array_equal(K32.cpu().numpy(), K.cpu().numpy().astype(np.float32))
 
 
 @pytest.mark.parametrize("elem_type", ["P1", "P2", "P4"])
@@ -423,14 +424,17 @@ def test_node_gather_kr_fold_vs_oracle(gpu, elem_type):
     assert _rel(K2.cpu().numpy(), exp0) < 1e-12
 
 
-@pytest.mark.parametrize("variant", [0, 1])
-def test_p2_element_kernel_variants_agree(gpu, variant):
-    """The direct and the pipelined element-matrix kernels give the oracle's regularised element
-    matrices on an odd element count (the persistent grid's last partial sweep)."""
-    print(f"case: P2 element kernel variant {variant}", flush=True)
+@pytest.mark.parametrize("rule", ["P2", "P4"])
+def test_p2_element_kernels_vs_oracle(gpu, rule):
+    """P2 element matrices with K_r folded, on an odd element count: the pipelined persistent
+    kernel (11-point rule; the last sweep of the grid is partial) and the one-wavefront-per-element
+    kernel (P2 elements at the 24 points of the P4 rule, as the P4 multigrid's P2 level) both give
+    the oracle's regularised element matrices."""
+    print(f"case: P2 element kernel, {rule} points", flush=True)
     from slope_stability_petsc4py_amd import _lib
+    from slope_stability_petsc4py_amd.fem import quadrature_3d
     m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=7.5, nz=3)
-    geom = fem.geometry(m.coord, m.elem, "P2")
+    geom = fem.geometry(m.coord, m.elem, "P2", quad=quadrature_3d(rule))
     n_e = geom.n_elem - 1 if geom.n_elem % 2 == 0 else geom.n_elem   # odd count
     dev = gpu
     rng = np.random.default_rng(43)
@@ -448,14 +452,10 @@ def test_p2_element_kernel_variants_agree(gpu, variant):
     wt, DS, G, lam, Del = geom.weight[:n_int], DS[:n_int], G[:n_int], lam[:n_int], Del[:n_int]
     ke = torch.full((n_e, 30, 30), np.nan, dtype=torch.float64, device=dev)
     args = [t(dphi), t(DS), t(wt), t(G), t(lam)]
-    _lib.call("ssx_set_element_kernel", variant, 16)
-    try:
-        _lib.call("ssx_element_matrices_3d", 10, int(args[0].data_ptr()), int(args[1].data_ptr()),
-                  int(args[2].data_ptr()), n_e, geom.n_q, r, int(args[3].data_ptr()), int(args[4].data_ptr()),
-                  int(ke.data_ptr()), 0)
-        torch.cuda.synchronize()
-    finally:
-        _lib.call("ssx_set_element_kernel", 1, 24)
+    _lib.call("ssx_element_matrices_3d", 10, int(args[0].data_ptr()), int(args[1].data_ptr()),
+              int(args[2].data_ptr()), n_e, geom.n_q, r, int(args[3].data_ptr()), int(args[4].data_ptr()),
+              int(ke.data_ptr()), 0)
+    torch.cuda.synchronize()
     exp = oracle.element_matrices_3d(dphi, (1 - r) * DS + r * Del, wt, n_e, geom.n_q)
     got = ke.cpu().numpy()
     assert np.isfinite(got).all()

@@ -235,17 +235,17 @@ def test_fused_chebyshev_step_matches_unfused(gpu, elastic3d, use32, degree):
     x0 = torch.from_numpy(rng.normal(size=K.n_rows)).to(gpu).to(dt)
     from slope_stability_petsc4py_amd import _lib
     out = {}
+    from gpu_helpers import misaligned
     for fused in (False, True, "vec4"):
-        lv = _Level(K, use32)
+        # fused=True: one entry per lane (arrays off the 16-byte grid); "vec4": aligned quadruples
+        lv = _Level(K if fused == "vec4" else misaligned(K), use32)
         lv.refresh()
         lv.fused = bool(fused)
-        _lib.call("ssx_set_spmv_kernel", 1 if fused == "vec4" else 0)    # fused=True: scalar rows
         xa = torch.empty_like(b)
         lv.cheb(b, xa, degree, 0.1, zero_guess=True)
         xb = x0.clone()
         lv.cheb(b, xb, degree, 0.1, zero_guess=False)
         out[fused] = (xa.double().cpu().numpy(), xb.double().cpu().numpy())
-    _lib.call("ssx_set_spmv_kernel", 1)
     tol = 1e-5 if use32 else 1e-12
     for i in range(2):
         ref = out[False][i]
