@@ -27,7 +27,8 @@ from .loads import strip_footing_load
 from .materials import MaterialSpec
 from .workloads import HETERO_LAYERS, HOMO
 
-DATA = Path(__file__).resolve().parents[1] / "tests" / "golden"
+# the reference's Gmsh tet4 meshes as arrays (written by tests/golden/make_golden.py)
+DATA = Path(__file__).resolve().parent / "data" / "meshes"
 
 HETERO_L1_MATERIALS = [
     MaterialSpec(15.0, 30.0, 0.0, 10000.0, 0.33, 19.0, 19.0),
@@ -75,7 +76,7 @@ def homo2d(h: float = 1.0, x1: float = 15.0, x2: float = 10.0, x3: float = 15.0)
 
 
 def gmsh_fixture_mesh(fixture: str, elem_type: str = "P2") -> M.SlopeMesh:
-    """A committed Gmsh-family mesh (tests/golden/*_tet4.npz): logical material and boundary ids
+    """A committed Gmsh-family mesh (data/meshes/*_tet4.npz): logical material and boundary ids
     from the file's physical names, Dirichlet labels x: [1, 2], y: [5], z: [3, 4] (the families'
     definition.py)."""
     d = np.load(DATA / fixture, allow_pickle=False)

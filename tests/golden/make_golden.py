@@ -9,7 +9,8 @@
 * ssr_histories.npz   -- data fields of the reference's committed SSR runs (lambda / omega / Umax
   histories of benchmarks/run_2D_homo_SSR_capture and benchmarks/run_3D_hetero_SSR_capture).
 * ll_histories.npz    -- the same fields of its committed limit-load runs (2D homo, 3D homo, 3D hetero).
-* *_tet4.npz          -- the reference's Gmsh tet4 meshes (SSR hetero L1/L2, LL homo/hetero L1) as arrays:
+* *_tet4.npz          -- the reference's Gmsh tet4 meshes (SSR hetero L1/L2, LL homo/hetero L1) as arrays,
+  written to slope_stability_petsc4py_amd/data/meshes/ (the package's cases load them):
   raw physical tags and the logical material / boundary ids their $PhysicalNames give
   ("material_<k>", "boundary_<k>", io.py:165-189), which the Dirichlet labels refer to.
 
@@ -24,6 +25,7 @@ from pathlib import Path
 import numpy as np
 
 ROOT = Path(__file__).resolve().parents[2]
+MESH_OUT = ROOT / "slope_stability_petsc4py_amd" / "data" / "meshes"
 sys.path.insert(0, str(ROOT))
 
 import oracle  # noqa: E402
@@ -106,7 +108,7 @@ def meshes() -> None:
         path = REF / "meshes" / rel
         coord, tet4, tet_phys, tri3, tri_phys = mesh.read_gmsh22(path)
         names = mesh.read_gmsh_physical_names(path)
-        np.savez_compressed(OUT / f"{name}.npz", coord=coord, tet4=tet4.astype(np.int32),
+        np.savez_compressed(MESH_OUT / f"{name}.npz", coord=coord, tet4=tet4.astype(np.int32),
                             tet_phys=tet_phys.astype(np.int8), tri3=tri3.astype(np.int32),
                             tri_phys=tri_phys.astype(np.int8),
                             tet_label=mesh.logical_ids(tet_phys, names, 3, "material").astype(np.int8),

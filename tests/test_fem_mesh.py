@@ -95,8 +95,8 @@ def test_tet10_elevation_first_encounter_order():
 def test_hetero_L1_known_answers(et, nodes, constrained, free):
     """Known answers of the reference's loader tests (tests/test_3d_hetero_ssr_gmsh_loader.py:40-93)
     and run sizes (80,362 P2 unknowns of the 3D hetero SSR capture; 616,322 P4 unknowns)."""
-    from pathlib import Path
-    d = np.load(Path(__file__).resolve().parent / "golden" / "ssr_hetero_L1_tet4.npz")
+    from slope_stability_petsc4py_amd.cases import DATA
+    d = np.load(DATA / "ssr_hetero_L1_tet4.npz")
     assert d["coord"].shape == (3, 3845) and d["tet4"].shape == (4, 18419) and d["tri3"].shape == (3, 6325)
     m = mesh.tet_mesh_from_arrays(d["coord"], d["tet4"].astype(np.int64), d["tet_phys"].astype(np.int64) - 1,
                                   d["tri3"].astype(np.int64), d["tri_phys"].astype(np.int64) - 1,

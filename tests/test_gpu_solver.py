@@ -321,3 +321,36 @@ def test_p4_ssr_matches_reference_history():
     assert lo < lam < hi
     assert abs(lam - interp) < 1e-3
     assert abs(res.lambda_hist[0] - 1.0) < 1e-12 and abs(res.lambda_hist[1] - 1.1) < 1e-12
+
+
+def test_footing_config_size_forms_agree(gpu):
+    """The strip-footing limit load at the size of the large LL configuration (528,504 P2 tets,
+    2,194,269 dofs; sparse P1 level above the aggregate level): plain FP64, FP32 and learned + FP32
+    forms followed to omega = 1000 give t at equal omega within 1e-3 of each other."""
+    import time
+    from pathlib import Path as _P
+    from slope_stability_petsc4py_amd.learn import LearnedPredictor, NodeMLP, node_info
+    weights = _P(cases.__file__).resolve().parent / "data" / "predictor_hetero3d_L1.npz"
+    case = cases.footing3d_hetero(h=0.53)
+    case.params.omega_max_stop = 1000.0
+    assert case.mesh.n_elem > 500_000
+    out = {}
+    for form in ("plain_fp64", "fp32", "learned_fp32"):
+        t0 = time.perf_counter()
+        s = cases.make_solver(case, precision="fp64" if form == "plain_fp64" else "fp32")
+        assert s.op.pattern.n_rows > 2_100_000 and s.linear.pc.coarse == "aggregate"
+        pred = None
+        if form == "learned_fp32":
+            pred = LearnedPredictor(NodeMLP.load(weights), node_info(case.mesh, case.materials))
+        res = cases.run_ll(s, predictor=pred)
+        out[form] = (np.asarray(res.omega_hist), np.asarray(res.t_hist))
+        print(f"case: footing config size {form}: t {res.t_hist[-1]:.6f} omega {res.omega_hist[-1]:.1f} "
+              f"steps {len(res.t_hist)} newton {res.newton_iterations} wall {res.wall_time:.1f}s "
+              f"total {time.perf_counter() - t0:.1f}s", flush=True)
+        del s, res
+        torch.cuda.empty_cache()
+    ref = out["plain_fp64"]
+    for form in ("fp32", "learned_fp32"):
+        w = min(ref[0][-1], out[form][0][-1])
+        t_ref = np.interp(w, *ref)
+        assert abs(np.interp(w, *out[form]) - t_ref) <= 1e-3 * t_ref, form
