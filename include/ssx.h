@@ -244,6 +244,16 @@ int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indic
 int ssx_sym_tiles_sizes(int64_t n, int64_t *tile_elems, int64_t *part_elems);
 /* Pack a symmetric n x n matrix (FP64 if a_f64, else FP32) into tiles (FP64 if t_f64, else FP32). */
 int ssx_sym_tiles_pack(int64_t n, const void *A, int a_f64, void *tiles, int t_f64, void *stream);
+/* Explicit inverse of a symmetric positive definite n x n matrix A (FP64, symmetric, either storage
+ * order) into the symmetric tile store above (tiles in FP64 if t_f64, else FP32), computed in FP64
+ * (f64 = 1) or FP32: blocked Cholesky A = L L^T on 64 x 64 tiles, W = L^-1, A^-1 = W^T W for the
+ * stored upper tiles; the matrix is padded to a multiple of 64 with the identity, and the tiles are
+ * zero outside n x n.  work: ssx_spd_inverse_sizes elements of the computation precision.  *info
+ * (device int) receives 0, or 1 + the index of the first non-positive pivot.  Replaces the dense
+ * coarse factorisation of the p-multigrid set-up (Hypre/PETSc there, linear/solver.py:466-513). */
+int ssx_spd_inverse_sizes(int64_t n, int64_t *work_elems);
+int ssx_spd_inverse_tiles(int64_t n, const double *A, int f64, void *work, void *tiles, int t_f64, int *info,
+                          void *stream);
 int ssx_sym_tiles_gemv_f32(int64_t n, const float *tiles, const float *x, float *part, float *y, void *stream);
 int ssx_sym_tiles_gemv_f64(int64_t n, const double *tiles, const double *x, double *part, double *y, void *stream);
 

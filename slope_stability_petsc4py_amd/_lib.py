@@ -43,6 +43,8 @@ _SIGS: dict[str, list] = {
     "ssx_cheb_step_f64": [I64, P, P, P, P, P, P, P, P, F64, F64, I32, I32, P],
     "ssx_sym_tiles_sizes": [I64, P, P],
     "ssx_sym_tiles_pack": [I64, P, I32, P, I32, P],
+    "ssx_spd_inverse_sizes": [I64, P],
+    "ssx_spd_inverse_tiles": [I64, P, I32, P, P, I32, P, P],
     "ssx_sym_tiles_gemv_f32": [I64, P, P, P, P, P],
     "ssx_sym_tiles_gemv_f64": [I64, P, P, P, P, P],
     "ssx_a_orthogonalize": [I64, I32, P, I64, P, P, P, I32, F64, P, P, P, P, P],

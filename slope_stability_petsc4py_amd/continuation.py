@@ -97,14 +97,17 @@ def init_phase_ssr(lambda_init, d_lambda_init, d_lambda_min, it_newt_max, it_dam
 def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda_diff_scaled_min, step_max,
                               omega_max_stop, it_newt_max, it_damp_max, tol, r_min, f, builder, solver, *,
                               free_mask, dim, predictor=None, store_step_u=False, progress=None,
-                              predictor_weights=(1.0, 0.5, 0.25), predictor_gate: str = "first_residual") -> SSRResult:
+                              predictor_weights=(1.0, 0.5, 0.25), predictor_gate: str = "first_residual",
+                              predictor_patience: int = 3) -> SSRResult:
     """SSR indirect continuation; returns the lambda/omega/Umax histories (FoS = lambda_hist[-1]).
 
     ``predictor(ctx) -> U_guess or None`` is the learned-predictor slot: ctx holds the accepted
     states, histories and the omega target; it returns U_guess or (U_guess, lambda_guess).  The
     guess is blended with the secant guess at the weights ``predictor_weights`` (1 = the guess
     itself), the blend with the smallest first residual (at lambda_guess, default the current
-    lambda) is tried first.  ``predictor_gate`` "first_residual": it is kept only if that residual
+    lambda) is tried first.  After ``predictor_patience`` learned guesses in a row were gated off or
+    rejected, the predictor is no longer called (its inference and the gate's residual evaluations
+    then cost nothing).  ``predictor_gate`` "first_residual": it is kept only if that residual
     is below the secant's; "newton_progress": Newton starts from it and the attempt is dropped
     (the secant then used) when the first Newton step does not reduce the residual."""
     t0 = perf_counter()
@@ -127,6 +130,7 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
     step = 2
     n_omega = 0
     attempts = 0
+    misses = 0          # learned guesses gated off or rejected in a row
     stop_reason = "completed"
     while True:
         omega_it = min(omega + d_omega, omega_max_stop)
@@ -135,6 +139,8 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
         U_sec = U + alpha_sec * (U - U_old)
         kind = "secant"
         learned = None
+        if predictor is not None and misses >= predictor_patience > 0:
+            predictor = None
         if predictor is not None:
             learned = predictor({"U": U, "U_old": U_old, "omega": omega, "omega_old": omega_old,
                                  "omega_target": omega_it, "lambda": lambda_value, "lambda_old": lambda_old,
@@ -186,6 +192,8 @@ def ssr_indirect_continuation(lambda_init, d_lambda_init, d_lambda_min, d_lambda
                 kind = "secant_after_rejection"
         U_it, lambda_cand, flag, it_newt, info = result
         solver.restore_deflation_basis(basis_snap)
+        if learned is not None:
+            misses = 0 if (kind == "learned" and flag == 0) else misses + 1
         attempts += 1
         rec = {"step": step + 1, "omega_target": omega_it, "flag": flag, "newton_it": it_newt, "predictor": kind,
                "attempt_wall": perf_counter() - t_att, "newton_total": stats.newton_iterations - newton_it_before,
@@ -277,7 +285,7 @@ class LoadOffsetBuilder:
 def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_stop, it_newt_max, it_damp_max, tol,
                              r_min, f, builder, solver, *, free_mask, dim, lambda_ell: float = 1.0,
                              progress=None, f_const: torch.Tensor | None = None, predictor=None,
-                             store_step_u: bool = False) -> LLResult:
+                             store_step_u: bool = False, predictor_patience: int = 3) -> LLResult:
     """Limit-load continuation in omega = f^T U (continuation/limit_load.py:66-374 with the driver's
     elastic start, cli/run_3D_hetero_SSR_capture.py:1224-1305): one elastic solve K_el U_el = f
     (strength reduced by lambda_ell), omega_el = f^T U_el, start from U = scale*U_el, t = 0, first
@@ -325,6 +333,7 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
     step_info = []
     step = 1
     n_omega = 0
+    misses = 0          # learned guesses gated off or rejected in a row (predictor_patience)
     stop_reason = "completed"
     while True:
         omega_it = omega + d_omega
@@ -338,6 +347,8 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
             U_ini = U
         kind = "secant"
         gate_info = {}
+        if predictor is not None and misses >= predictor_patience > 0:
+            predictor = None
         if predictor is not None and step > 1:
             guess = predictor({"U": U, "U_old": U_old, "omega": omega, "omega_old": omega_old,
                                "omega_target": omega_it, "lambda": t, "states": states})
@@ -363,6 +374,8 @@ def ll_indirect_continuation(d_omega_ini_scale, d_t_min, step_max, omega_max_sto
             U_it, t_it, flag, it_newt = newton_ind_ll(U_sec, t, omega_it, it_newt_max, it_damp_max, tol, r_min, f,
                                                       builder, solver, free_mask=free_mask, stats=stats)
             kind = "secant_after_rejection"
+        if kind != "secant":
+            misses = 0 if kind == "learned" else misses + 1
         solver.restore_deflation_basis(basis_snap)
         step_info.append({"step": step, "omega_target": omega_it, "flag": flag, "newton_it": it_newt,
                           "predictor": kind, **gate_info})

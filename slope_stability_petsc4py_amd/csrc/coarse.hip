@@ -138,6 +138,257 @@ int sym_gemv(int64_t n, const T *tiles, const T *x, T *part, T *y, void *stream)
     return SSX_OK;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Explicit inverse of the symmetric positive definite coarse operator, straight into the symmetric
+// tile store above: A = L L^T (blocked right-looking Cholesky on 64 x 64 tiles), W = L^-1 (blocked
+// right-looking triangular inverse), then A^-1 = W^T W for the upper tiles only.  About n^3 flops
+// in all (n^3/3 per stage), in T = float or double, every tile product done by one workgroup from
+// LDS (tile_mm).  The matrix is padded to a multiple of 64 with an identity block (its inverse is
+// then the identity there, and the stored tiles are zeroed outside n x n).
+//
+// Work (elements of T): M (n_pad^2, the operator, overwritten by L), W (n_pad^2), and the inverses
+// of the 64 x 64 diagonal blocks of L (nb tiles).  `info` receives 0, or 1 + the index of the
+// first non-positive pivot (the caller then retries in FP64 or with a diagonal shift).
+// Replaces the dense factorisation of the coarse level (Hypre/PETSc coarse solvers there,
+// linear/solver.py:466-513).
+// ---------------------------------------------------------------------------------------------
+constexpr int kH = 32;            // kk extent of one LDS staging pass of tile_mm
+
+// acc (4 x 4 per thread, rows ty + 16 i, columns tx + 16 j, ty = tid / 16, tx = tid % 16) +=
+// (SUB: -=) op(A) op(B) for one 64 x 64 x 64 tile product; A, B point at tile origins with
+// leading dimensions lda, ldb; op(A)[r][kk] = TA ? A[kk][r] : A[r][kk], op(B)[kk][c] =
+// TB ? B[c][kk] : B[kk][c].  All 256 threads call it; it stages kH values of kk at a time.
+template <typename T, bool TA, bool TB, bool SUB>
+__device__ __forceinline__ void tile_mm(T (&acc)[4][4], const T *__restrict__ A, int64_t lda,
+                                        const T *__restrict__ B, int64_t ldb, T *As, T *Bs) {
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (int h = 0; h < kT; h += kH) {
+        __syncthreads();   // the previous pass's reads are done
+        for (int e = tid; e < kH * kT; e += 256) {
+            int kk, r;
+            if constexpr (TA) { r = e & (kT - 1); kk = e >> 6; }
+            else { kk = e & (kH - 1); r = e >> 5; }
+            As[kk * kTP + r] = TA ? A[(int64_t)(h + kk) * lda + r] : A[(int64_t)r * lda + h + kk];
+            int kb, c;
+            if constexpr (TB) { kb = e & (kH - 1); c = e >> 5; }
+            else { c = e & (kT - 1); kb = e >> 6; }
+            Bs[kb * kTP + c] = TB ? B[(int64_t)c * ldb + h + kb] : B[(int64_t)(h + kb) * ldb + c];
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int kk = 0; kk < kH; ++kk) {
+            T a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = As[kk * kTP + ty + 16 * i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = Bs[kk * kTP + tx + 16 * j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = SUB ? fma(-a[i], b[j], acc[i][j]) : fma(a[i], b[j], acc[i][j]);
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_load(T (&acc)[4][4], const T *__restrict__ C, int64_t ld) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = C[(int64_t)(ty + 16 * i) * ld + tx + 16 * j];
+}
+
+template <typename T>
+__device__ __forceinline__ void tile_store(const T (&acc)[4][4], T *__restrict__ C, int64_t ld) {
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) C[(int64_t)(ty + 16 * i) * ld + tx + 16 * j] = acc[i][j];
+}
+
+// M (n_pad x n_pad, row-major) = A (n x n, FP64, symmetric) padded with the identity; W = identity.
+template <typename T>
+__global__ void k_spd_load(int64_t n, int64_t n_pad, const double *__restrict__ A, T *__restrict__ M,
+                           T *__restrict__ W, int *__restrict__ info) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g == 0) *info = 0;
+    if (g >= n_pad * n_pad) return;
+    const int64_t i = g / n_pad, j = g - (g / n_pad) * n_pad;
+    const bool in = i < n && j < n;
+    M[g] = in ? (T)A[in ? j * n + i : 0] : (T)(i == j ? 1 : 0);
+    W[g] = (T)(i == j ? 1 : 0);
+}
+
+// Diagonal block k: L_kk (Cholesky in LDS, written to M) and its inverse (to Dinv[k]).
+template <typename T>
+__global__ void __launch_bounds__(256) k_spd_potrf(int64_t n_pad, int k, T *__restrict__ M, T *__restrict__ Dinv,
+                                                  int *__restrict__ info) {
+    __shared__ T S[kT * kTP];
+    const int tid = threadIdx.x;
+    T *Mk = M + (int64_t)k * kT * n_pad + (int64_t)k * kT;
+    for (int e = tid; e < kT * kT; e += 256) S[(e >> 6) * kTP + (e & 63)] = Mk[(int64_t)(e >> 6) * n_pad + (e & 63)];
+    for (int j = 0; j < kT; ++j) {
+        __syncthreads();
+        if (tid == 0) {
+            T d = S[j * kTP + j];
+            if (!(d > (T)0)) {
+                if (*info == 0) *info = k * kT + j + 1;
+                d = (T)1;
+            }
+            S[j * kTP + j] = sqrt(d);
+        }
+        __syncthreads();
+        const T djj = S[j * kTP + j];
+        if (tid > j && tid < kT) S[tid * kTP + j] /= djj;
+        __syncthreads();
+        for (int e = tid; e < kT * kT; e += 256) {
+            const int i = e >> 6, l = e & 63;
+            if (l > j && l <= i) S[i * kTP + l] -= S[i * kTP + j] * S[l * kTP + j];
+        }
+    }
+    __syncthreads();
+    for (int e = tid; e < kT * kT; e += 256) {
+        const int i = e >> 6, l = e & 63;
+        Mk[(int64_t)i * n_pad + l] = l <= i ? S[i * kTP + l] : (T)0;
+    }
+    // in-place inverse of the lower triangle, columns right to left: with the trailing block
+    // already inverted (X22), column j of the inverse is -X22 L[j+1:, j] / L[j][j]
+    for (int j = kT - 1; j >= 0; --j) {
+        __syncthreads();
+        const T inv = (T)1 / S[j * kTP + j];
+        T y = 0;
+        if (tid > j && tid < kT)
+            for (int m = j + 1; m <= tid; ++m) y += S[tid * kTP + m] * S[m * kTP + j];
+        __syncthreads();
+        if (tid > j && tid < kT) S[tid * kTP + j] = -inv * y;
+        if (tid == 0) S[j * kTP + j] = inv;
+    }
+    __syncthreads();
+    T *D = Dinv + (int64_t)k * kT * kT;
+    for (int e = tid; e < kT * kT; e += 256) {
+        const int i = e >> 6, l = e & 63;
+        D[e] = l <= i ? S[i * kTP + l] : (T)0;
+    }
+}
+
+// Panel: L_ik = A_ik L_kk^-T for block rows i > k.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spd_trsm(int64_t n_pad, int k, T *__restrict__ M, const T *__restrict__ Dinv) {
+    __shared__ T As[kH * kTP], Bs[kH * kTP];
+    const int64_t i = (int64_t)k + 1 + blockIdx.x;
+    T *Aik = M + i * kT * n_pad + (int64_t)k * kT;
+    T acc[4][4] = {};
+    tile_mm<T, false, true, false>(acc, Aik, n_pad, Dinv + (int64_t)k * kT * kT, kT, As, Bs);
+    tile_store(acc, Aik, n_pad);
+}
+
+// lower-triangle tile index t -> (i, j), 0 <= j <= i
+__device__ __forceinline__ void lower_coords(int64_t t, int64_t &i, int64_t &j) {
+    int64_t r = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((r + 1) * (r + 2) / 2 <= t) ++r;
+    while (r * (r + 1) / 2 > t) --r;
+    i = r;
+    j = t - r * (r + 1) / 2;
+}
+
+// Trailing update: A_ij -= L_ik L_jk^T for k < j <= i.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spd_syrk(int64_t n_pad, int k, T *__restrict__ M) {
+    __shared__ T As[kH * kTP], Bs[kH * kTP];
+    int64_t ii, jj;
+    lower_coords(blockIdx.x, ii, jj);
+    const int64_t i = k + 1 + ii, j = k + 1 + jj;
+    T *Cij = M + i * kT * n_pad + j * kT;
+    T acc[4][4];
+    tile_load(acc, Cij, n_pad);
+    tile_mm<T, false, true, true>(acc, M + i * kT * n_pad + (int64_t)k * kT, n_pad,
+                                  M + j * kT * n_pad + (int64_t)k * kT, n_pad, As, Bs);
+    tile_store(acc, Cij, n_pad);
+}
+
+// Triangular inverse, step k (W starts as the identity): row block k, W_kj = L_kk^-1 W_kj (j <= k) ...
+template <typename T>
+__global__ void __launch_bounds__(256) k_spd_trtri_row(int64_t n_pad, int k, T *__restrict__ W,
+                                                      const T *__restrict__ Dinv) {
+    __shared__ T As[kH * kTP], Bs[kH * kTP];
+    T *Wkj = W + (int64_t)k * kT * n_pad + (int64_t)blockIdx.x * kT;
+    T acc[4][4] = {};
+    tile_mm<T, false, false, false>(acc, Dinv + (int64_t)k * kT * kT, kT, Wkj, n_pad, As, Bs);
+    tile_store(acc, Wkj, n_pad);
+}
+
+// ... then W_ij -= L_ik W_kj for i > k, j <= k.
+template <typename T>
+__global__ void __launch_bounds__(256) k_spd_trtri_update(int64_t n_pad, int k, const T *__restrict__ M,
+                                                         T *__restrict__ W) {
+    __shared__ T As[kH * kTP], Bs[kH * kTP];
+    const int64_t i = (int64_t)k + 1 + blockIdx.x / (k + 1), j = blockIdx.x % (k + 1);
+    T *Wij = W + i * kT * n_pad + j * kT;
+    T acc[4][4];
+    tile_load(acc, Wij, n_pad);
+    tile_mm<T, false, false, true>(acc, M + i * kT * n_pad + (int64_t)k * kT, n_pad,
+                                   W + (int64_t)k * kT * n_pad + j * kT, n_pad, As, Bs);
+    tile_store(acc, Wij, n_pad);
+}
+
+// A^-1 = W^T W: the stored upper tile (I, J), I <= J, is sum_{k >= J} W_kI^T W_kJ, written to the
+// symmetric tile store (zero outside n x n).
+template <typename T, typename Tout>
+__global__ void __launch_bounds__(256) k_spd_lauum_pack(int64_t n, int64_t n_pad, int nb, const T *__restrict__ W,
+                                                       Tout *__restrict__ tiles) {
+    __shared__ T As[kH * kTP], Bs[kH * kTP];
+    int64_t I, J;
+    tile_coords(blockIdx.x, I, J);
+    T acc[4][4] = {};
+    for (int64_t k = J; k < nb; ++k)
+        tile_mm<T, true, false, false>(acc, W + k * kT * n_pad + I * kT, n_pad, W + k * kT * n_pad + J * kT, n_pad,
+                                       As, Bs);
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    Tout *out = tiles + (int64_t)blockIdx.x * kT * kT;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = ty + 16 * i, c = tx + 16 * j;
+            const bool in = I * kT + r < n && J * kT + c < n;
+            out[r * kT + c] = in ? (Tout)acc[i][j] : (Tout)0;
+        }
+}
+
+template <typename T>
+int spd_inverse(int64_t n, const double *A, T *work, void *tiles, bool t_f64, int *info, hipStream_t st) {
+    const int64_t nb = (n + kT - 1) / kT, n_pad = nb * kT;
+    T *M = work, *W = work + n_pad * n_pad, *Dinv = work + 2 * n_pad * n_pad;
+    hipLaunchKernelGGL((k_spd_load<T>), dim3(ssx::grid_for(n_pad * n_pad, 256)), dim3(256), 0, st, n, n_pad, A, M,
+                       W, info);
+    for (int k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL((k_spd_potrf<T>), dim3(1), dim3(256), 0, st, n_pad, k, M, Dinv, info);
+        const int64_t m = nb - k - 1;
+        if (m > 0) {
+            hipLaunchKernelGGL((k_spd_trsm<T>), dim3((unsigned)m), dim3(256), 0, st, n_pad, k, M, Dinv);
+            hipLaunchKernelGGL((k_spd_syrk<T>), dim3((unsigned)(m * (m + 1) / 2)), dim3(256), 0, st, n_pad, k, M);
+        }
+    }
+    for (int k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL((k_spd_trtri_row<T>), dim3((unsigned)(k + 1)), dim3(256), 0, st, n_pad, k, W, Dinv);
+        const int64_t m = nb - k - 1;
+        if (m > 0)
+            hipLaunchKernelGGL((k_spd_trtri_update<T>), dim3((unsigned)(m * (k + 1))), dim3(256), 0, st, n_pad, k, M,
+                               W);
+    }
+    const unsigned n_tiles = (unsigned)(nb * (nb + 1) / 2);
+    if (t_f64)
+        hipLaunchKernelGGL((k_spd_lauum_pack<T, double>), dim3(n_tiles), dim3(256), 0, st, n, n_pad, (int)nb, W,
+                           (double *)tiles);
+    else
+        hipLaunchKernelGGL((k_spd_lauum_pack<T, float>), dim3(n_tiles), dim3(256), 0, st, n, n_pad, (int)nb, W,
+                           (float *)tiles);
+    SSX_CHECK_LAUNCH("ssx_spd_inverse_tiles");
+    return SSX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -169,6 +420,25 @@ int ssx_sym_tiles_pack(int64_t n, const void *A, int a_f64, void *tiles, int t_f
         hipLaunchKernelGGL((k_sym_pack<float, double>), grid, block, 0, st, n, n_tiles, (const float *)A, (double *)tiles);
     SSX_CHECK_LAUNCH("ssx_sym_tiles_pack");
     return SSX_OK;
+}
+
+int ssx_spd_inverse_sizes(int64_t n, int64_t *work_elems) {
+    SSX_REQUIRE(n >= 0 && work_elems, "ssx_spd_inverse_sizes: bad arguments");
+    const int64_t nb = (n + kT - 1) / kT, n_pad = nb * kT;
+    *work_elems = 2 * n_pad * n_pad + nb * kT * kT;
+    return SSX_OK;
+}
+
+int ssx_spd_inverse_tiles(int64_t n, const double *A, int f64, void *work, void *tiles, int t_f64, int *info,
+                          void *stream) {
+    SSX_REQUIRE(n >= 0, "ssx_spd_inverse_tiles: n < 0");
+    if (n == 0) return SSX_OK;
+    SSX_REQUIRE(A && work && tiles && info, "ssx_spd_inverse_tiles: null pointer");
+    const int64_t nb = (n + kT - 1) / kT;
+    SSX_REQUIRE(nb * (nb + 1) / 2 <= 0x7fffffff && nb * nb <= 0x7fffffff, "ssx_spd_inverse_tiles: n too large");
+    hipStream_t st = ssx::as_stream(stream);
+    if (f64) return spd_inverse<double>(n, A, (double *)work, tiles, t_f64 != 0, info, st);
+    return spd_inverse<float>(n, A, (float *)work, tiles, t_f64 != 0, info, st);
 }
 
 int ssx_sym_tiles_gemv_f32(int64_t n, const float *tiles, const float *x, float *part, float *y, void *stream) {

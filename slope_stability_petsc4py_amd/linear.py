@@ -69,23 +69,12 @@ def axpby(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> None:
         _lib.call("ssx_axpby_f32", y.numel(), float(a), ptr(x), float(b), ptr(y), _st())
 
 
-def lower_triangular_inverse(L: torch.Tensor, base: int = 512) -> torch.Tensor:
-    """Inverse of a lower-triangular matrix by recursive 2x2 blocks:
-    [[A, 0], [C, D]]^-1 = [[A^-1, 0], [-D^-1 C A^-1, D^-1]]; blocks of at most `base` rows by a
-    triangular solve against the identity."""
-    n = L.shape[0]
-    if n <= base:
-        return torch.linalg.solve_triangular(L, torch.eye(n, dtype=L.dtype, device=L.device), upper=False)
-    k = ((n // 2 + 63) // 64) * 64          # split on a multiple of 64 rows when that is a split
-    if k >= n:
-        k = n // 2
-    Ai = lower_triangular_inverse(L[:k, :k], base)
-    Di = lower_triangular_inverse(L[k:, k:], base)
-    out = torch.zeros_like(L)
-    out[:k, :k] = Ai
-    out[k:, k:] = Di
-    out[k:, :k] = -(Di @ (L[k:, :k] @ Ai))
-    return out
+def spd_inverse_work(n: int) -> int:
+    """Scratch elements (of the computation precision) of `ssx_spd_inverse_tiles` for n x n."""
+    import ctypes
+    out = ctypes.c_int64(0)
+    _lib.call("ssx_spd_inverse_sizes", int(n), ctypes.addressof(out))
+    return int(out.value)
 
 
 def sym_tile_sizes(n: int) -> tuple[int, int]:
@@ -606,7 +595,7 @@ class PMultigrid:
         # precision of the coarse factorisation: FP64, or FP32 (falls back to FP64 when the FP32
         # Cholesky meets a non-positive pivot); default follows the level precision
         self.coarse_precision = coarse_precision or ("fp32" if use32 else "fp64")
-        self.timings = {"level_refresh": 0.0, "galerkin": 0.0, "factor": 0.0, "inverse": 0.0, "coarse_builds": 0,
+        self.timings = {"level_refresh": 0.0, "galerkin": 0.0, "factor": 0.0, "coarse_builds": 0,
                         "coarse_fp64_fallbacks": 0}
         self._setups = 0
         p_ptr, p_idx, p_w, n_c, c_free = p2_to_p1_prolongation(elem, q_mask)
@@ -652,6 +641,8 @@ class PMultigrid:
         self.level1 = None
         self.Ainv = None          # packed symmetric tiles of the coarse inverse
         self._tile_part = None
+        self._spd_work = None
+        self._spd_info = None
         self._dev_ready = False
         self._wb = {}
 
@@ -753,6 +744,20 @@ class PMultigrid:
             self._stale = False
         self._setups += 1
 
+    def _spd_inverse(self, Ac: torch.Tensor, f64: bool, dt) -> bool:
+        """Ainv tiles <- Ac^-1 (`ssx_spd_inverse_tiles`, computed in FP64 or FP32); False when the
+        factorisation met a non-positive pivot."""
+        n = self.n_d
+        need = spd_inverse_work(n) * (8 if f64 else 4)
+        if self._spd_work is None or self._spd_work.numel() < need:
+            self._spd_work = None
+            self._spd_work = torch.empty(need, dtype=torch.uint8, device=Ac.device)
+        if self._spd_info is None:
+            self._spd_info = torch.zeros(1, dtype=torch.int32, device=Ac.device)
+        _lib.call("ssx_spd_inverse_tiles", n, ptr(Ac), int(f64), ptr(self._spd_work), ptr(self.Ainv),
+                  int(dt == torch.float64), ptr(self._spd_info), _st())
+        return int(self._spd_info.item()) == 0
+
     def note_iterations(self, its: int) -> None:
         """Called by the Krylov solver after each solve with its iteration count.  The coarse
         operator is marked stale when a solve needs more than `refresh_iterations` iterations and
@@ -778,57 +783,35 @@ class PMultigrid:
         torch.cuda.synchronize()
         t1 = perf_counter()
         tm["galerkin"] += t1 - t0
-        L = None
-        if self.coarse_precision == "fp32":
-            L32, info = torch.linalg.cholesky_ex(Ac.to(torch.float32))
-            if int(info.item()) == 0:
-                L = L32
-            else:
-                tm["coarse_fp64_fallbacks"] += 1
-            del L32
-        if L is None:
-            L, info = torch.linalg.cholesky_ex(Ac)
-            if int(info.item()) != 0:
-                # indefinite coarse operator (strong softening): shifted factorisation, the shift
-                # raised by 100x until the factorisation succeeds (the coarse level only has to be
-                # a preconditioner for the outer FGMRES)
-                dmax = float(Ac.diagonal().abs().max().item()) or 1.0
-                for rel_shift in (1e-8, 1e-6, 1e-4, 1e-2, 1.0):
-                    Ac.diagonal().add_(rel_shift * dmax)
-                    L, info = torch.linalg.cholesky_ex(Ac)
-                    tm["coarse_shifts"] = tm.get("coarse_shifts", 0) + 1
-                    if int(info.item()) == 0:
-                        break
-                else:
-                    raise RuntimeError("p-multigrid: coarse operator not positive definite after shifting")
-        del Ac
-        torch.cuda.synchronize()
-        t2 = perf_counter()
-        tm["factor"] += t2 - t1
-        # A^-1: the library's Cholesky inverse while its right-hand side block (n x n) stays below
-        # 1.5 GB (above about 2 GB the solver library fails with an internal error); beyond that
-        # L^-T L^-1 with L^-1 by recursive 2x2 block inversion (GEMMs, diagonal blocks of <= 512 rows
-        # by triangular solves -- a large triangular solve fails to get its BLAS workspace)
-        n_l = L.shape[0]
-        if n_l * n_l * L.element_size() < 1.5 * 2 ** 30:
-            Ainv = torch.cholesky_inverse(L).contiguous()
-            del L
-        else:
-            Linv = lower_triangular_inverse(L)
-            del L
-            Ainv = (Linv.mT @ Linv).contiguous()
-            del Linv
-        # the inverse is kept as the upper triangle of 64 x 64 tiles in the level precision
+        # explicit inverse in the symmetric tile store (the coarse solve is then one tile GEMV):
+        # blocked Cholesky + triangular inverse + W^T W on the device, in the coarse precision; an
+        # FP32 factorisation that meets a non-positive pivot is redone in FP64, and an indefinite
+        # operator (strong softening) gets a diagonal shift raised by 100x until it factorises (the
+        # coarse level only has to be a preconditioner for the outer FGMRES)
         dt = torch.float32 if self.use32 else torch.float64
         te, pe = sym_tile_sizes(self.n_d)
         if self._tile_part is None or self._tile_part.dtype != dt:
             self._tile_part = torch.empty(pe, dtype=dt, device=dev)
         self.Ainv = torch.empty(te, dtype=dt, device=dev)
-        _lib.call("ssx_sym_tiles_pack", self.n_d, ptr(Ainv), int(Ainv.dtype == torch.float64), ptr(self.Ainv),
-                  int(dt == torch.float64), _st())
-        del Ainv
+        ok = False
+        if self.coarse_precision == "fp32":
+            ok = self._spd_inverse(Ac, False, dt)
+            if not ok:
+                tm["coarse_fp64_fallbacks"] += 1
+        if not ok:
+            ok = self._spd_inverse(Ac, True, dt)
+        if not ok:
+            dmax = float(Ac.diagonal().abs().max().item()) or 1.0
+            for rel_shift in (1e-8, 1e-6, 1e-4, 1e-2, 1.0):
+                Ac.diagonal().add_(rel_shift * dmax)
+                tm["coarse_shifts"] = tm.get("coarse_shifts", 0) + 1
+                if self._spd_inverse(Ac, True, dt):
+                    break
+            else:
+                raise RuntimeError("p-multigrid: coarse operator not positive definite after shifting")
+        del Ac
         torch.cuda.synchronize()
-        tm["inverse"] += perf_counter() - t2
+        tm["factor"] += perf_counter() - t1
         tm["coarse_builds"] += 1
 
     def cycle(self, b: torch.Tensor, x: torch.Tensor) -> None:

@@ -1,7 +1,9 @@
 """Command-line runner: ``python -m slope_stability_petsc4py_amd.run --case homo2d --precision fp64``.
 
 Runs one SSR (or LL) continuation on the device and prints one JSON line with the factor of
-safety (or limit load), the histories, wall time and iteration counts.
+safety (or limit load), the histories, wall time and iteration counts; with ``--out DIR`` it also
+writes the reference's run artefacts (petsc_run.npz, run_info.json; ``--store-steps`` adds the
+accepted states step_U).
 """
 
 from __future__ import annotations
@@ -41,6 +43,8 @@ def main(argv=None):
     ap.add_argument("--coarse", default="auto", choices=["auto", "dense", "aggregate"],
                     help="p-multigrid coarse level: dense P1, or sparse P1 above a dense aggregate level "
                          "(auto: aggregate above 50000 free P1 dofs)")
+    ap.add_argument("--out", default=None, help="write petsc_run.npz and run_info.json into this directory")
+    ap.add_argument("--store-steps", action="store_true", help="keep the accepted states (step_U) in --out")
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
@@ -78,7 +82,7 @@ def main(argv=None):
                   file=sys.stderr, flush=True)
 
     if a.analysis == "ssr":
-        res = cases.run_ssr(s, progress=prog)
+        res = cases.run_ssr(s, progress=prog, store_step_u=a.store_steps)
         out = {"case": case.name, "precision": a.precision, "pc": a.pc, "fos": res.fos,
                "lambda_hist": res.lambda_hist.tolist(), "omega_hist": res.omega_hist.tolist(),
                "Umax_hist": res.Umax_hist.tolist(), "stop_reason": res.stop_reason,
@@ -90,7 +94,7 @@ def main(argv=None):
                "coarse": getattr(s.linear.pc, "coarse", None), "n_coarse_dense": getattr(s.linear.pc, "n_d", None),
                "timings": s.op.get_total_time()}
     else:
-        res = cases.run_ll(s, progress=prog)
+        res = cases.run_ll(s, progress=prog, store_step_u=a.store_steps)
         out = {"case": case.name, "precision": a.precision, "limit_load": res.limit_load,
                "t_hist": res.t_hist.tolist(), "omega_hist": res.omega_hist.tolist(),
                "Umax_hist": res.Umax_hist.tolist(), "wall_time_s": res.wall_time, "setup_s": t_setup,
@@ -99,6 +103,9 @@ def main(argv=None):
                "n_elem": int(case.mesh.n_elem), "linear_solver": s.linear.iteration_collector.summary(),
                "pc_timings": getattr(s.linear.pc, "timings", None), "extra": case.extra,
                "coarse": getattr(s.linear.pc, "coarse", None), "n_coarse_dense": getattr(s.linear.pc, "n_d", None)}
+    if a.out:
+        from .runio import write_run
+        write_run(a.out, case, s, res, analysis=a.analysis, precision=a.precision, setup_s=t_setup)
     print(json.dumps(out), flush=True)
 
 

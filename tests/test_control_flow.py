@@ -188,6 +188,14 @@ def test_learned_predictor_gate_rejects_bad_guess():
     assert calls["n"] > 0
     kinds = {r["predictor"] for r in res.step_info}
     assert "learned" not in kinds and "learned_gated" in kinds
+    # patience: after 3 guesses in a row gated off, the predictor is not called again
+    assert calls["n"] == 3 and sum(r["predictor"] == "learned_gated" for r in res.step_info) == 3
+    assert len(res.step_info) > 3
+    calls["n"] = 0
+    continuation.ssr_indirect_continuation(
+        1.0, 0.5, 1e-5, 1e-3, 8, 1e9, 50, 10, 1e-9, 1e-4, f, TanhBuilder(), DiagSolver(),
+        free_mask=torch.ones(n, dtype=torch.float64), dim=3, predictor=bad_predictor, predictor_patience=0)
+    assert calls["n"] == len(res.step_info)      # patience 0: called at every attempt
 
 
 def test_learned_predictor_exact_guess_is_used():
@@ -260,20 +268,6 @@ def test_load_offset_builder_subtracts_constant_load():
     assert torch.equal(b.build_F_reduced_vec(U), torch.tensor([5.5, 9.0]))
     assert torch.equal(b.build_F_all_vec(2.0, U), torch.tensor([5.5, 9.0]))
     assert b.reduction(1.5) == "ok" and fb.calls == [1.5]
-
-
-def test_lower_triangular_inverse_recursive_blocks():
-    """Recursive 2x2-block inverse of a lower-triangular factor (the coarse-level inverse) == the
-    identity check L^-1 L = I, stays lower triangular, for sizes around the block boundaries."""
-    import torch
-    from slope_stability_petsc4py_amd.linear import lower_triangular_inverse
-    for n in (1, 63, 64, 65, 300):
-        rng = np.random.default_rng(n)
-        M = rng.normal(size=(n, n))
-        L = torch.linalg.cholesky(torch.from_numpy(M @ M.T + n * np.eye(n)))
-        Li = lower_triangular_inverse(L, base=32)
-        assert torch.allclose(Li @ L, torch.eye(n, dtype=torch.float64), atol=1e-12)
-        assert torch.all(torch.triu(Li, 1) == 0)
 
 
 def _ll_tanh_case():

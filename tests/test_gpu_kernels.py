@@ -496,6 +496,46 @@ def test_sym_tiles_gemv_vs_numpy(gpu, n, f64):
         assert _rel(y.cpu().numpy().astype(np.float64), exp) < 2e-6
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 300, 1000])
+@pytest.mark.parametrize("f64", [False, True])
+def test_spd_inverse_tiles_vs_numpy(gpu, n, f64):
+    """Explicit inverse of an SPD matrix (blocked Cholesky, triangular inverse, W^T W on 64 x 64
+    tiles) into the symmetric tile store: the coarse solve with it equals numpy.linalg.solve; the
+    computation and the stored tiles in FP64 or FP32; repeated calls give the same bits."""
+    from slope_stability_petsc4py_amd.device import ptr
+    from slope_stability_petsc4py_amd.linear import spd_inverse_work, sym_tile_sizes
+    print(f"case: spd inverse n={n} f64={f64}", flush=True)
+    rng = np.random.default_rng(7 + n)
+    Q, _ = np.linalg.qr(rng.normal(size=(n, n)))
+    A = (Q * np.logspace(0, 4, n)) @ Q.T           # condition number 1e4
+    A = 0.5 * (A + A.T)
+    x = rng.normal(size=n)
+    dt = torch.float64 if f64 else torch.float32
+    te, pe = sym_tile_sizes(n)
+    work = torch.empty(spd_inverse_work(n), dtype=dt, device=gpu)
+    info = torch.full((1,), -7, dtype=torch.int32, device=gpu)
+    tiles = torch.full((te,), np.nan, dtype=dt, device=gpu)
+    Ad = torch.from_numpy(A).to(gpu)
+    _lib.call("ssx_spd_inverse_tiles", n, ptr(Ad), int(f64), ptr(work), ptr(tiles), int(f64), ptr(info), 0)
+    assert int(info.item()) == 0
+    part = torch.empty(pe, dtype=dt, device=gpu)
+    y = torch.empty(n, dtype=dt, device=gpu)
+    _lib.call("ssx_sym_tiles_gemv_f64" if f64 else "ssx_sym_tiles_gemv_f32", n, ptr(tiles),
+              ptr(torch.from_numpy(x).to(dt).to(gpu)), ptr(part), ptr(y), 0)
+    exp = np.linalg.solve(A, x)
+    assert _rel(y.cpu().numpy().astype(np.float64), exp) < (1e-10 if f64 else 2e-3)
+    tiles2 = torch.empty_like(tiles)
+    _lib.call("ssx_spd_inverse_tiles", n, ptr(Ad), int(f64), ptr(work), ptr(tiles2), int(f64), ptr(info), 0)
+    assert torch.equal(tiles, tiles2)
+    if n >= 64:
+        # an indefinite matrix reports its first non-positive pivot
+        B = A.copy()
+        B[n // 2, n // 2] = -1e6
+        _lib.call("ssx_spd_inverse_tiles", n, ptr(torch.from_numpy(B).to(gpu)), int(f64), ptr(work), ptr(tiles2),
+                  int(f64), ptr(info), 0)
+        assert 0 < int(info.item()) <= n // 2 + 1
+
+
 @pytest.mark.parametrize("f64", [False, True])
 def test_inv_diag_vs_scipy(gpu, asm3d, f64):
     """Inverse diagonal of the CSR operator (Jacobi/Chebyshev smoother) == 1 / diag, 0 on a zero

@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from oracle import mlp as O
-from slope_stability_petsc4py_amd.learn import NodeMLP
+from slope_stability_petsc4py_amd.learn import NodeMLP, train
 
 pytestmark = pytest.mark.gpu
 
@@ -72,17 +72,17 @@ def test_adam_matches_oracle(gpu):
     np.testing.assert_allclose(net.W[0].cpu().numpy(), w1, rtol=1e-5, atol=1e-6)
 
 
-def test_learned_predictor_gain_on_hetero_ssr(gpu):
-    """What the trained network gains over the method without it: the network is trained (device
-    forward/backward/Adam) on the continuation histories of three perturbed-material runs of the
-    3D heterogeneous L1 slope, then fills the predictor slot of the SSR continuation on the
-    reference materials (FP32 form), against the same continuation with the secant predictor.
-    The Newton-iteration and wall-time ratios are printed (the gain); the factor of safety must
-    not move by more than 1e-3, and the first-residual gate must keep the learned form within 10%
-    of the plain form's Newton iterations."""
+def test_learned_predictor_keeps_fos_and_bounds_cost_on_hetero_ssr(gpu):
+    """The network is trained (device forward/backward/Adam) on the continuation histories of
+    three perturbed-material runs of the 3D heterogeneous L1 slope (the loss must fall), then fills
+    the predictor slot of the SSR continuation on the reference materials (FP32 form).  Checked:
+    the factor of safety stays within 1e-3 of the secant form's and of the reference's, and the
+    first-residual gate keeps the Newton iterations within 10% (+2) of the secant form's.  The
+    Newton and wall-time ratios (what the network gains or costs here) are printed, not asserted:
+    on this slope the gate keeps most steps on the secant."""
     from slope_stability_petsc4py_amd import cases
     from slope_stability_petsc4py_amd.learn import LearnedPredictor, node_info, samples_from_history, train
-    from slope_stability_petsc4py_amd.learn_experiment import hetero_variant
+    from slope_stability_petsc4py_amd.learned_slopes import hetero_variant
     print("case: learned predictor on hetero3d_L1", flush=True)
     samples = []
     for seed in range(3):
@@ -109,3 +109,24 @@ def test_learned_predictor_gain_on_hetero_ssr(gpu):
     assert abs(lrn.fos - plain.fos) < 1e-3
     assert abs(lrn.fos - 1.6661099309952119) < 1e-3
     assert lrn.newton_iterations <= 1.1 * plain.newton_iterations + 2
+
+
+def test_learned_predictor_on_seeded_homogeneous_slopes(gpu):
+    """The learned-predictor configuration at a reduced size: seeded homogeneous slopes (height,
+    face angle, c, phi drawn per seed in the configuration's ranges, h = 2 m), the network trained on
+    six seeds' own SSR histories, evaluated on two held-out seeds against the plain (secant) method:
+    the factor of safety within 1e-3 of the plain method's on every held-out seed; the Newton and
+    wall-time ratios are printed."""
+    from slope_stability_petsc4py_amd.learned_slopes import collect, evaluate, slope_case, summary
+    print("case: learned predictor on seeded homogeneous slopes", flush=True)
+    case_of = lambda seed: slope_case(seed, h=2.0)  # noqa: E731
+    samples, runs = collect(case_of, range(6), log=lambda *a: None)
+    assert len(samples) >= 30 and all(r["steps"] >= 5 for r in runs)
+    net = NodeMLP(seed=0)
+    losses = train(net, samples, epochs=30, lr=2e-3)
+    assert losses[-1] < losses[0]
+    evals = evaluate(net, case_of, [1000, 1001], log=lambda *a: None)
+    sm = summary(evals)
+    print("held-out:", [(e["seed"], e["newton_plain"], e["newton_learned"], round(e["fos_abs_diff"], 6))
+                        for e in evals], "summary", sm, flush=True)
+    assert all(e["fos_abs_diff"] < 1e-3 for e in evals)
