@@ -12,8 +12,11 @@ One step = the Newton-iteration hot path on those inputs:
 of the step (the tangent) against the 8 TB/s HBM peak, timed with HIP events on its launch stream,
 with the HBM bytes per launch read by rocprofv3 (profiles/traffic.json).  `cpu_baseline` is the
 reference's own C kernels (built from its sources into oracle/_ref, OpenMP on the host cores) on
-the same arrays.  `ssr` is the end-to-end leg of the metric: the reference's 3D heterogeneous P2
-SSR to the factor of safety, plain FP64 against the FP32 and learned + FP32 forms.
+the same arrays.  `p4` is the matrix-core leg: the heterogeneous P4 slope (33,935 tets, 1.14M dofs),
+return mapping + FP64 MFMA element matrices + node gather, with its own `roofline` (MFMA).  `ssr` is
+the end-to-end leg of the metric: the reference's 3D heterogeneous P2 SSR to the factor of safety,
+plain FP64 against the FP32 and learned + FP32 forms; `ssr.ssr2d` is the reference's 2D
+homogeneous SSR on the GPU beside its CPU restatement (oracle/ssr2d.py, one host core).
 
 Prints one JSON line.  Rank/world handling: one GPU only (--gpus > 1 exits with a message).
 """
@@ -165,8 +168,17 @@ def run_gpu(args):
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
+        # second and third calibrations, other access kinds: a read-only reduction of the same
+        # 256 MiB, and a gather of 128-byte rows in random order (whole cache lines, as the node
+        # gathers read them; 128 MiB read, 128 MiB written)
+        for _ in range(3):
+            torch.sum(src, dtype=torch.float64)
+        rows = src.view(-1, 16)
+        idx = torch.randperm(rows.shape[0], device=dev)[: rows.shape[0] // 2]
+        for _ in range(3):
+            rows.index_select(0, idx)
         torch.cuda.synchronize()
-        del src, dst
+        del src, dst, rows, idx
     ab = algorithmic_bytes(op, fold=mat is not None)
     kern = {}
     for name, lst in ev.items():
@@ -446,7 +458,7 @@ def run_p4(args):
         },
         "cpu_baseline": None,
     }
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -465,6 +477,7 @@ def main():
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run a 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--no-ssr", action="store_true", help="skip the end-to-end SSR leg")
+    ap.add_argument("--no-p4", action="store_true", help="skip the P4 matrix-core leg")
     ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows"],
                     help="P2 tangent path: element matrices + node gather (default) or the fused row kernel")
     args = ap.parse_args()
@@ -473,7 +486,7 @@ def main():
               "GPUs is a design, not an implementation.", file=sys.stderr)
         sys.exit(2)
     if args.config == "p4":
-        run_p4(args)
+        print(json.dumps(run_p4(args)), flush=True)
         return
     res = run_gpu(args)
     op = res["op"]
@@ -485,6 +498,11 @@ def main():
     if not args.no_cpu:
         _log("cpu baseline (reference C on host cores)...")
         cpu = cpu_baseline(op, res["E"], res["ab"], args.cpu_budget)
+    p4 = None
+    if not args.no_p4:
+        _log("P4 leg (matrix-core element matrices)...")
+        r4 = run_p4(args)
+        p4 = {k: r4[k] for k in ("value", "unit", "ms_per_step", "config", "roofline", "kernels")}
     ssr = None
     if not args.no_ssr:
         _log("end-to-end SSR leg (3D heterogeneous P2)...")
@@ -525,6 +543,7 @@ def main():
                                "measured in this run") if res["traffic"] is not None else None,
         },
         "cpu_baseline": cpu,
+        "p4": p4,
         "ssr": ssr,
         "kernels": kern,
         "host": {"cpu_model": cpu_model, "nproc": os.cpu_count(), "python": platform.python_version()},

@@ -185,14 +185,16 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
  * node's dim rows over its elements (node slots ns_ptr/ns_elem/ns_lnode, element order) at the
  * int16 column offsets ns_off (n_slots x nld, relative to the row start, -1 = constrained column),
  * sets constrained rows to their unit diagonal, optionally blends (1-r) K + r elastic, and writes
- * FP64 values and an optional FP32 copy.  max_len = longest free row (< 32768).  dim 3, nld 12/30/105.
+ * FP64 values and an optional FP32 copy.  nodes = the node ids to gather (n_nodes of them; NULL:
+ * nodes 0 .. n_nodes-1), max_len = longest free row among them (< 32768; it sizes the LDS of a
+ * workgroup, so callers group nodes of similar row length).  dim 3, nld 12/30/105.
  * Together with ssx_element_matrices_3d / _p4 replaces assemble_tangent_values_3d_rows_c
  * (cython/assemble_tangent_values_3d.c:111-214) and the constrained-diagonal step of
  * assemble_owned_tangent_values (fem/distributed_tangent.py:1559-1612). */
 int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_ptr, const int32_t *ns_elem,
                          const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
-                         const uint8_t *row_free, int64_t n_nodes, int max_len, double r, const double *elastic,
-                         double *out, float *out32, void *stream);
+                         const uint8_t *row_free, const int32_t *nodes, int64_t n_nodes, int max_len, double r,
+                         const double *elastic, double *out, float *out32, void *stream);
 
 /* ---- learned continuation predictor (no reference counterpart: it fills the predictor slot of
  * continuation/indirect.py:2703-2830, where the reference uses the secant of :550-566) ---- */

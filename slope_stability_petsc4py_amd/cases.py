@@ -91,6 +91,14 @@ def hetero3d_L1(elem_type: str = "P2") -> Case:
                 SSRParams())
 
 
+def hetero3d_L2_layers(elem_type: str = "P4") -> Case:
+    """The heterogeneous P4 configuration: the reference's SSR_hetero_ada_L2 mesh (33,935 tets;
+    elevated to P4: 1.14M dofs) with the four horizontal layer materials HETERO_LAYERS on its four
+    material labels, SSR parameters of the reference's heterogeneous capture."""
+    return Case("hetero3d_L2_layers", gmsh_fixture_mesh("ssr_hetero_L2_tet4.npz", elem_type), list(HETERO_LAYERS),
+                SSRParams())
+
+
 HOMO_LL_MATERIAL = MaterialSpec(6.0, 45.0, 0.0, 40000.0, 0.30, 20.0, 20.0)
 
 

@@ -313,13 +313,14 @@ __global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ 
                                                      const uint8_t *__restrict__ ns_lnode,
                                                      const int16_t *__restrict__ ns_off,
                                                      const int32_t *__restrict__ indptr,
-                                                     const uint8_t *__restrict__ row_free, double r,
+                                                     const uint8_t *__restrict__ row_free,
+                                                     const int32_t *__restrict__ nodes, double r,
                                                      const double *__restrict__ elastic, double *__restrict__ out,
                                                      float *__restrict__ out32) {
     constexpr int NV = DIM * NLD;
     constexpr int NPASS = (NV + 63) / 64;
     extern __shared__ double acc[];
-    const int64_t node = blockIdx.x;
+    const int64_t node = nodes ? (int64_t)nodes[blockIdx.x] : (int64_t)blockIdx.x;
     const int lane = threadIdx.x;
     const int64_t r0 = DIM * node;
     int fr[DIM];
@@ -403,8 +404,9 @@ __global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ 
 
 template <int DIM, int NLD>
 int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_elem, const uint8_t *ns_lnode,
-                  const int16_t *ns_off, const int32_t *indptr, const uint8_t *row_free, int64_t n_nodes,
-                  int max_len, double r, const double *elastic, double *out, float *out32, hipStream_t st) {
+                  const int16_t *ns_off, const int32_t *indptr, const uint8_t *row_free, const int32_t *nodes,
+                  int64_t n_nodes, int max_len, double r, const double *elastic, double *out, float *out32,
+                  hipStream_t st) {
     const size_t lds = ((size_t)DIM * max_len + 1) * sizeof(double);
     if (lds > 65536) {
         static int max_lds = -1;
@@ -423,7 +425,7 @@ int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_ele
         }
     }
     hipLaunchKernelGGL((k_gather_nodes<DIM, NLD>), dim3((unsigned)n_nodes), dim3(64), lds, st, ke, ns_ptr, ns_elem,
-                       ns_lnode, ns_off, indptr, row_free, r, elastic, out, out32);
+                       ns_lnode, ns_off, indptr, row_free, nodes, r, elastic, out, out32);
     SSX_CHECK_LAUNCH("ssx_gather_node_rows");
     return SSX_OK;
 }
@@ -482,8 +484,8 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
 
 int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_ptr, const int32_t *ns_elem,
                          const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
-                         const uint8_t *row_free, int64_t n_nodes, int max_len, double r, const double *elastic,
-                         double *out, float *out32, void *stream) {
+                         const uint8_t *row_free, const int32_t *nodes, int64_t n_nodes, int max_len, double r,
+                         const double *elastic, double *out, float *out32, void *stream) {
     SSX_REQUIRE(n_nodes >= 0 && max_len >= 0 && max_len < 32768, "ssx_gather_node_rows: bad sizes");
     if (n_nodes == 0) return SSX_OK;
     SSX_REQUIRE(ke && ns_ptr && ns_elem && ns_lnode && ns_off && indptr && row_free && out,
@@ -491,7 +493,8 @@ int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_p
     SSX_REQUIRE(n_nodes <= 0x7fffffff, "ssx_gather_node_rows: too many nodes for one launch");
     hipStream_t st = ssx::as_stream(stream);
 #define CALL(D, N) \
-    launch_gather<D, N>(ke, ns_ptr, ns_elem, ns_lnode, ns_off, indptr, row_free, n_nodes, max_len, r, elastic, out, out32, st)
+    launch_gather<D, N>(ke, ns_ptr, ns_elem, ns_lnode, ns_off, indptr, row_free, nodes, n_nodes, max_len, r, elastic, \
+                        out, out32, st)
     if (dim == 3 && nld == 12) return CALL(3, 12);
     if (dim == 3 && nld == 30) return CALL(3, 30);
     if (dim == 3 && nld == 105) return CALL(3, 105);

@@ -2,11 +2,11 @@
 //
 // ke = sum_q B_q^T (w_q D_q) B_q is one GEMM per element, ke (105 x 105) = A (105 x 144) * Bm (144 x 105)
 // with K = 24 points x 6 strains, A[i][k] = B_q[s][i] and Bm[k][j] = sum_t w_q D_q[s][t] B_q[t][j]
-// (k = 6 q + s).  One wavefront per element; the 105 x 105 result is padded to 7 x 7 tiles of
-// 16 x 16 and each tile accumulates 36 v_mfma_f64_16x16x4 steps.  Both operands are formed per lane
-// from the element's dphi and w*DS staged in LDS (B is sparse: every column of B_q has 3 nonzeros,
-// so an operand value costs <= 3 FMAs).  DS is symmetric for associated flow, so only the upper
-// tile triangle (28 of 49 tiles) is computed and mirrored on write.
+// (k = 6 q + s).  The 105 x 105 result is padded to 7 x 7 tiles of 16 x 16 and each tile
+// accumulates 36 v_mfma_f64_16x16x4 steps.  Both operands are formed from the element's dphi and
+// w*DS staged in LDS (B is sparse: every column of B_q has 3 nonzeros, so an operand value costs
+// <= 3 FMAs).  DS is symmetric for associated flow, so only the upper tile triangle (28 of 49
+// tiles) is computed and mirrored on write.
 //
 // The element matrices are then gathered per mesh node into the fixed CSR pattern
 // (ssx_gather_node_rows in node_assembly.hip), in the reference's `rows` summation order
@@ -48,17 +48,34 @@ __device__ __forceinline__ double pick6(int s, const double c[6]) {
     return v;
 }
 
-__global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dphi, const double *__restrict__ DS,
-                                                   const double *__restrict__ w, int64_t n_elem, int64_t n_int,
-                                                   double r, const double *__restrict__ shear,
-                                                   const double *__restrict__ lame, double *__restrict__ ke) {
-    __shared__ double sg[3][NQ4][NP4];  // gradients of the element's basis functions
-    __shared__ double sd[NQ4][36];      // w_q * DS_q (column-major 6x6)
+// One workgroup of four wavefronts per element.  The Bm operand (w D_r B, 144 x 112 padded) is
+// formed once per element into LDS, in three chunks of KC = 48 k values (8 points), fragment-major:
+// sb[tj][ks][lane] is the value lane `lane` feeds v_mfma_f64_16x16x4 at k-step ks of tile column
+// tj (k = 4 ks + lane / 16, j = 16 tj + lane % 16), so each MFMA reads one contiguous 512-byte
+// fragment.  The A operand (B^T) costs a few VALU operations per k-step and tile row, and is formed
+// in registers.  The 28 upper tiles are split 7 per wavefront: wavefront w takes tile row w (tiles
+// w..6) and tile row 7 - w (tiles 7 - w..6), so its slot t is (w, w + t) for t < 7 - w, else
+// (7 - w, t).  LDS: 20,160 B gradients + 6,912 B w D + 43,008 B fragments = 70,080 B, so two
+// elements (eight wavefronts) share a CU and one element's operand formation overlaps the other's
+// MFMAs.
+constexpr int NPAD = 16 * TILES;   // 112
+constexpr int KC = 48;             // k values per LDS chunk (8 points x 6 strains)
+constexpr int KSC = KC / 4;        // MFMA k-steps per chunk
+constexpr int NCHUNK = KDIM / KC;  // 3
+
+__global__ void __launch_bounds__(256, 2) k_ke_p4_mfma(const double *__restrict__ dphi, const double *__restrict__ DS,
+                                                      const double *__restrict__ w, int64_t n_elem, int64_t n_int,
+                                                      double r, const double *__restrict__ shear,
+                                                      const double *__restrict__ lame, double *__restrict__ ke) {
+    __shared__ double sg[3][NQ4][NP4];        // gradients of the element's basis functions
+    __shared__ double sd[NQ4][36];            // w_q * D_q (column-major 6x6)
+    __shared__ double sb[TILES][KSC][64];     // Bm fragments of one chunk
     const int64_t e = blockIdx.x;
     if (e >= n_elem) return;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & 63;
     const int64_t g0 = e * NQ4;
-    for (int t = lane; t < 3 * NQ4 * NP4; t += 64) {
+    for (int t = tid; t < 3 * NQ4 * NP4; t += 256) {
         const int c = t / (NQ4 * NP4), rem = t % (NQ4 * NP4);
         sg[c][rem / NP4][rem % NP4] = dphi[((int64_t)c * n_int + g0) * NP4 + rem];
     }
@@ -66,7 +83,7 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
     // D_el = lame * vol vol^T + 2 G * ident (ident = diag(1, 1, 1, 1/2, 1/2, 1/2)) when the
     // regularisation is folded in (shear != nullptr)
     const bool fold = shear != nullptr;
-    for (int t = lane; t < NQ4 * 36; t += 64) {
+    for (int t = tid; t < NQ4 * 36; t += 256) {
         const int q = t / 36, k = t % 36;
         const int i = k % 6, j = k / 6;
         double d = DS[(g0 + q) * 36 + k];
@@ -77,60 +94,74 @@ __global__ void __launch_bounds__(64) k_ke_p4_mfma(const double *__restrict__ dp
         }
         sd[q][k] = w[g0 + q] * d;
     }
-    __syncthreads();
 
-    const int col = lane & 15;   // i for A, j for B and D
+    const int col = lane & 15;   // i for A, j for B
     const int kq = lane >> 4;    // k offset within the 4-wide step
-    double *out = ke + e * (int64_t)(NLD4 * NLD4);
-    for (int ti = 0; ti < TILES; ++ti) {
-        d4 acc[TILES];
+    const int tiA = wv, tiB = TILES - wv;          // tile rows of this wavefront (tiB = 7: none)
+    const int nA = TILES - wv;                     // slots of tile row tiA
+    // A-operand rows of the two tile rows (padded rows >= 105 use a valid node and a zero weight)
+    const int iaA = 16 * tiA + col, iaB = 16 * (tiB < TILES ? tiB : 0) + col;
+    const double vaA = iaA < NLD4 ? 1.0 : 0.0, vaB = (tiB < TILES && iaB < NLD4) ? 1.0 : 0.0;
+    const int iacA = iaA < NLD4 ? iaA : NLD4 - 1, iacB = iaB < NLD4 ? iaB : NLD4 - 1;
+    const int naA = iacA / 3, daA = iacA % 3, naB = iacB / 3, daB = iacB % 3;
+
+    d4 acc[TILES];
 #pragma unroll
-        for (int tj = 0; tj < TILES; ++tj) acc[tj] = d4{0.0, 0.0, 0.0, 0.0};
-        const int ia = 16 * ti + col;
-        // padded rows/columns (>= 105) use a valid node and a zero weight
-        const double va = ia < NLD4 ? 1.0 : 0.0;
-        const int iac = ia < NLD4 ? ia : NLD4 - 1;
-        const int na = iac / 3, da = iac % 3;
-        for (int ks = 0; ks < KDIM / 4; ++ks) {
-            const int k = 4 * ks + kq;
-            const int q = k / 6, s = k % 6;
-            double ca[6];
-            bcol(da, sg[0][q][na], sg[1][q][na], sg[2][q][na], ca);
-            const double a = va * pick6(s, ca);
+    for (int t = 0; t < TILES; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    double *sbf = &sb[0][0][0];
+    for (int ch = 0; ch < NCHUNK; ++ch) {
+        __syncthreads();   // staging done (first chunk) / the previous chunk's fragments are read
+        // Bm fragments of this chunk: element f = (tj * KSC + ksl) * 64 + fl
+        for (int f = tid; f < TILES * KSC * 64; f += 256) {
+            const int fl = f & 63, tk = f >> 6;
+            const int tj = tk / KSC, ksl = tk - (tk / KSC) * KSC;
+            const int k = ch * KC + 4 * ksl + (fl >> 4);
+            const int q = k / 6, s = k - (k / 6) * 6;
+            const int jb = 16 * tj + (fl & 15);
+            const double vb = jb < NLD4 ? 1.0 : 0.0;
+            const int jbc = jb < NLD4 ? jb : NLD4 - 1;
+            const int nb = jbc / 3, db = jbc % 3;
+            double cb[6];
+            bcol(db, sg[0][q][nb], sg[1][q][nb], sg[2][q][nb], cb);
+            // row s of (wD) B: sum_t wD[s][t] B[t][jb], D column-major (D[s + 6 t])
             const double *D = sd[q];
-            // row s of wD: D column-major, D[s + 6 t]
-            double drow[6];
+            double b = D[s] * cb[0];
 #pragma unroll
-            for (int t = 0; t < 6; ++t) drow[t] = D[s + 6 * t];
+            for (int t = 1; t < 6; ++t) b += D[s + 6 * t] * cb[t];
+            sbf[f] = vb * b;
+        }
+        __syncthreads();
+        for (int ksl = 0; ksl < KSC; ++ksl) {
+            const int k = ch * KC + 4 * ksl + kq;
+            const int q = k / 6, s = k - (k / 6) * 6;
+            double ca[6];
+            bcol(daA, sg[0][q][naA], sg[1][q][naA], sg[2][q][naA], ca);
+            const double aA = vaA * pick6(s, ca);
+            bcol(daB, sg[0][q][naB], sg[1][q][naB], sg[2][q][naB], ca);
+            const double aB = vaB * pick6(s, ca);
 #pragma unroll
-            for (int tj = 0; tj < TILES; ++tj) {
-                if (tj < ti) continue;
-                const int jb = 16 * tj + col;
-                const double vb = jb < NLD4 ? 1.0 : 0.0;
-                const int jbc = jb < NLD4 ? jb : NLD4 - 1;
-                const int nb = jbc / 3, db = jbc % 3;
-                double cb[6];
-                bcol(db, sg[0][q][nb], sg[1][q][nb], sg[2][q][nb], cb);
-                // row s of (wD) B: sum_t wD[s][t] B[t][jb]
-                double b = drow[0] * cb[0];
-#pragma unroll
-                for (int t = 1; t < 6; ++t) b += drow[t] * cb[t];
-                acc[tj] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, vb * b, acc[tj], 0, 0, 0);
+            for (int t = 0; t < TILES; ++t) {
+                const bool first = t < nA;                // wavefront-uniform
+                const int tj = first ? tiA + t : t;
+                const double b = sbf[(tj * KSC + ksl) * 64 + lane];
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(first ? aA : aB, b, acc[t], 0, 0, 0);
             }
         }
-        // accumulator layout of v_mfma_f64_16x16x4f64: register r of lane l holds row
-        // (l / 16) + 4 r of column l % 16
+    }
+    // accumulator layout of v_mfma_f64_16x16x4f64: register r of lane l holds row
+    // (l / 16) + 4 r of column l % 16
+    double *out = ke + e * (int64_t)(NLD4 * NLD4);
 #pragma unroll
-        for (int tj = 0; tj < TILES; ++tj) {
-            if (tj < ti) continue;
-            const int j = 16 * tj + col;
+    for (int t = 0; t < TILES; ++t) {
+        const bool first = t < nA;
+        const int ti = first ? tiA : tiB, tj = first ? tiA + t : t;
+        const int j = 16 * tj + col;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = 16 * ti + kq + 4 * r;
-                if (i < NLD4 && j < NLD4) {
-                    out[(int64_t)i * NLD4 + j] = acc[tj][r];
-                    if (tj > ti) out[(int64_t)j * NLD4 + i] = acc[tj][r];
-                }
+        for (int rr = 0; rr < 4; ++rr) {
+            const int i = 16 * ti + kq + 4 * rr;
+            if (i < NLD4 && j < NLD4) {
+                out[(int64_t)i * NLD4 + j] = acc[t][rr];
+                if (tj > ti) out[(int64_t)j * NLD4 + i] = acc[t][rr];
             }
         }
     }
@@ -146,7 +177,7 @@ int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *
     if (n_elem == 0) return SSX_OK;
     SSX_REQUIRE(dphi && DS && w && ke, "ssx_element_matrices_p4: null pointer");
     SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_matrices_p4: shear and lame go together");
-    hipLaunchKernelGGL(k_ke_p4_mfma, dim3((unsigned)n_elem), dim3(64), 0, ssx::as_stream(stream), dphi, DS, w, n_elem,
+    hipLaunchKernelGGL(k_ke_p4_mfma, dim3((unsigned)n_elem), dim3(256), 0, ssx::as_stream(stream), dphi, DS, w, n_elem,
                        n_elem * NQ4, r, shear, lame, ke);
     SSX_CHECK_LAUNCH("ssx_element_matrices_p4");
     return SSX_OK;

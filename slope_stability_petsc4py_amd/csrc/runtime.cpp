@@ -22,7 +22,7 @@ void set_error(const char *fmt, ...) {
 extern "C" {
 
 // ABI version of this library: bumped whenever an entry point's signature changes.
-int ssx_version(void) { return 3; }
+int ssx_version(void) { return 4; }
 
 const char *ssx_last_error(void) { return g_err; }
 

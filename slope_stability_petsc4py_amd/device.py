@@ -134,6 +134,10 @@ class DevicePattern:
             self.ns_elem = to_dev(ns.elem)
             self.ns_lnode = to_dev(ns.lnode)
             self.ns_off = to_dev(ns.off)
+            # one gather launch per row-length class when the rows are long (P4: a vertex row holds
+            # several times the columns of a face or interior node's row)
+            groups = ns.buckets() if ns.max_len > 192 else []
+            self.ns_groups = [(to_dev(ids), ml) for ids, ml in groups] if len(groups) > 1 else []
         self._ke = None
         self.force_gather = "nodes"     # "nodes" (node slots, 3D) or "rows" (row slots)
 
@@ -189,9 +193,12 @@ class DevicePattern:
 
     def gather_nodes(self, ke: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
                      elastic: torch.Tensor | None = None) -> None:
-        _lib.call("ssx_gather_node_rows", self.dim, self.nld, ptr(ke), ptr(self.ns_ptr), ptr(self.ns_elem),
-                  ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), self.host.n_nodes,
-                  self.ns.max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
+        groups = self.ns_groups or [(None, self.ns.max_len)]
+        for ids, max_len in groups:
+            n = self.host.n_nodes if ids is None else ids.numel()
+            _lib.call("ssx_gather_node_rows", self.dim, self.nld, ptr(ke), ptr(self.ns_ptr), ptr(self.ns_elem),
+                      ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), ptr(ids), n,
+                      max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
 
     # 3D tetrahedra: "nodes" (element matrices + node gather, the default) or "rows" (fused row
     # kernel, the 2D path)

@@ -141,6 +141,7 @@ def main(argv=None):
     ap.add_argument("--init", default=None,
                     help="continue training from these weights (training over shards, one call per shard)")
     ap.add_argument("--test", type=int, default=20)
+    ap.add_argument("--test-start", type=int, default=1000, help="first held-out seed")
     ap.add_argument("--h", type=float, default=HOMO_H)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=2e-3)
@@ -153,7 +154,7 @@ def main(argv=None):
     log = lambda *x: print(*x, file=sys.stderr, flush=True)  # noqa: E731
     if a.case == "homo":
         case_of = lambda seed: slope_case(seed, a.h)  # noqa: E731
-        test_seeds = range(1000, 1000 + a.test)
+        test_seeds = range(a.test_start, a.test_start + a.test)
     else:
         case_of = hetero_variant
         test_seeds = [-1]

@@ -198,6 +198,19 @@ class NodeSlots:
     lnode: np.ndarray      # (n_slots,) uint8
     off: np.ndarray        # (n_slots, dim*n_p) int16
     max_len: int           # longest free row
+    node_len: np.ndarray   # (n_nodes,) int32: length of the node's free rows (0: no free row)
+
+    def buckets(self, base: int = 96) -> list[tuple[np.ndarray, int]]:
+        """Nodes grouped by row length, (node ids, longest row of the group): lengths up to `base`,
+        then doubling classes.  The gather's LDS per workgroup is sized by the group's longest row,
+        so short-row nodes (most nodes of a P4 mesh) are not held to the longest row's LDS."""
+        out, lo, hi = [], -1, base
+        while lo < self.max_len:
+            ids = np.flatnonzero((self.node_len > lo) & (self.node_len <= hi)).astype(np.int32)
+            if ids.size:
+                out.append((ids, int(self.node_len[ids].max())))
+            lo, hi = hi, 2 * hi
+        return out
 
 
 def node_slots(pat: OwnedPattern) -> NodeSlots:
@@ -226,4 +239,5 @@ def node_slots(pat: OwnedPattern) -> NodeSlots:
         raise AssertionError("node-slot column offset outside its row")
     return NodeSlots(ptr=ptr.astype(np.int32), elem=pat.slot_elem[sel].astype(np.int32),
                      lnode=(pat.slot_lrow[sel] // dim).astype(np.uint8),
-                     off=np.ascontiguousarray(off.astype(np.int16).reshape(n, nld)), max_len=max_len)
+                     off=np.ascontiguousarray(off.astype(np.int16).reshape(n, nld)), max_len=max_len,
+                     node_len=np.where(has_free, row_len[r0], 0).astype(np.int32))

@@ -21,14 +21,16 @@ from . import cases
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--case", default="homo2d", choices=["homo2d", "hetero3d_L1", "slope3d_homo", "slope3d_hetero",
+    ap.add_argument("--case", default="homo2d", choices=["homo2d", "hetero3d_L1", "hetero3d_L2_layers",
+                                                         "slope3d_homo", "slope3d_hetero",
                                                          "homo2d_ll", "homo3d_ll_L1", "hetero3d_ll_L1",
                                                          "footing2d_prandtl", "footing3d_hetero"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--pc", default="pmg", choices=["pmg", "cheb"])
     ap.add_argument("--analysis", default="ssr", choices=["ssr", "ll"])
-    ap.add_argument("--elem-type", default="P2", choices=["P1", "P2", "P4"],
-                    help="element order of the hetero3d_L1 case (P4: the reference's 616,322-dof P4 problem)")
+    ap.add_argument("--elem-type", default=None, choices=["P1", "P2", "P4"],
+                    help="element order of the Gmsh-mesh cases (default P2; P4 for hetero3d_L2_layers); "
+                         "hetero3d_L1 with P4 is the reference's 616,322-dof P4 problem")
     ap.add_argument("--h", type=float, default=None)
     ap.add_argument("--nz", type=int, default=None)
     ap.add_argument("--step-max", type=int, default=None)
@@ -48,6 +50,8 @@ def main(argv=None):
     ap.add_argument("--verbose", action="store_true")
     a = ap.parse_args(argv)
     t0 = time.perf_counter()
+    if a.elem_type is None and a.case != "hetero3d_L2_layers":
+        a.elem_type = "P2"
     if a.case.endswith("_ll") or a.case.endswith("_ll_L1") or a.case.startswith("footing"):
         a.analysis = "ll"
     if a.case == "homo2d":
@@ -60,6 +64,8 @@ def main(argv=None):
         case = cases.hetero3d_ll_L1(a.elem_type)
     elif a.case == "hetero3d_L1":
         case = cases.hetero3d_L1(a.elem_type)
+    elif a.case == "hetero3d_L2_layers":
+        case = cases.hetero3d_L2_layers(a.elem_type or "P4")
     elif a.case == "footing2d_prandtl":
         case = cases.footing2d_prandtl(h=a.h or 0.25)
     elif a.case == "footing3d_hetero":
@@ -81,6 +87,10 @@ def main(argv=None):
             print(json.dumps({k: (float(v) if isinstance(v, (np.floating, float)) else v) for k, v in ev.items()}),
                   file=sys.stderr, flush=True)
 
+    pc = s.linear.pc
+    inner = pc.coarse if hasattr(getattr(pc, "coarse", None), "n_d") else pc   # P4: its P2 multigrid
+    coarse_kind = getattr(inner, "coarse", None)
+    n_coarse = getattr(inner, "n_d", None)
     if a.analysis == "ssr":
         res = cases.run_ssr(s, progress=prog, store_step_u=a.store_steps)
         out = {"case": case.name, "precision": a.precision, "pc": a.pc, "fos": res.fos,
@@ -91,8 +101,7 @@ def main(argv=None):
                "n_dof": int(s.op.pattern.n_rows), "n_elem": int(case.mesh.n_elem),
                "coarse_refresh": a.coarse_refresh, "linear_solver": s.linear.iteration_collector.summary(),
                "pc_timings": getattr(s.linear.pc, "timings", None),
-               "coarse": getattr(s.linear.pc, "coarse", None), "n_coarse_dense": getattr(s.linear.pc, "n_d", None),
-               "timings": s.op.get_total_time()}
+               "coarse": coarse_kind, "n_coarse_dense": n_coarse, "timings": s.op.get_total_time()}
     else:
         res = cases.run_ll(s, progress=prog, store_step_u=a.store_steps)
         out = {"case": case.name, "precision": a.precision, "limit_load": res.limit_load,
@@ -102,7 +111,7 @@ def main(argv=None):
                "linear_iterations": res.linear_iterations, "n_dof": int(s.op.pattern.n_rows),
                "n_elem": int(case.mesh.n_elem), "linear_solver": s.linear.iteration_collector.summary(),
                "pc_timings": getattr(s.linear.pc, "timings", None), "extra": case.extra,
-               "coarse": getattr(s.linear.pc, "coarse", None), "n_coarse_dense": getattr(s.linear.pc, "n_d", None)}
+               "coarse": coarse_kind, "n_coarse_dense": n_coarse}
     if a.out:
         from .runio import write_run
         write_run(a.out, case, s, res, analysis=a.analysis, precision=a.precision, setup_s=t_setup)
