@@ -206,6 +206,51 @@ def run_gpu(args):
     }
 
 
+def run_probes() -> dict:
+    """What this GPU attains, beside the published peaks: a device copy and a STREAM triad
+    (a = b + s c) over 1 GiB arrays, and the FP64 vector-FMA and FP64 matrix-core rates
+    (`ssx_probe_fp64`)."""
+    import ctypes
+
+    import torch
+
+    from slope_stability_petsc4py_amd import _lib
+    from slope_stability_petsc4py_amd.device import ptr, stream_ptr
+
+    dev = torch.device("cuda", 0)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(reps):
+            fn()
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) * 1e-3 / reps
+
+    n = 1 << 27                      # 1 GiB of FP64 per array
+    x = torch.ones(n, dtype=torch.float64, device=dev)
+    y = torch.empty_like(x)
+    z = torch.full_like(x, 2.0)
+    t_copy = timed(lambda: y.copy_(x), 10)
+    t_triad = timed(lambda: torch.add(x, z, alpha=3.0, out=y), 10)
+    del x, y, z
+    props = torch.cuda.get_device_properties(dev)
+    blocks = props.multi_processor_count * 8
+    out = torch.empty(blocks * 256, dtype=torch.float64, device=dev)
+    flops = ctypes.c_double(0.0)
+    rates = {}
+    for mode, name, iters in ((0, "fp64_fma", 20000), (1, "fp64_mfma", 4000)):
+        t = timed(lambda: _lib.call("ssx_probe_fp64", mode, blocks, iters, ptr(out), ctypes.addressof(flops),
+                                    stream_ptr()), 3)
+        rates[name] = flops.value / t / 1e12
+    return {"hbm_copy_GB_s": round(2 * 8 * n / t_copy / 1e9, 1), "hbm_triad_GB_s": round(3 * 8 * n / t_triad / 1e9, 1),
+            "fp64_fma_TFLOP_s": round(rates["fp64_fma"], 2), "fp64_mfma_TFLOP_s": round(rates["fp64_mfma"], 2),
+            "published": {"hbm_GB_s": HBM_PEAK_GBS, "fp64_vector_TFLOP_s": 78.6, "fp64_matrix_TFLOP_s": 78.6}}
+
+
 def cpu_baseline(op, E_host, ab, budget_s: float):
     """The reference's own C kernels (compiled from its sources into oracle/_ref) on the host cores:
     return mapping + force rows + tangent rows over a bounded sample of the same workload."""
@@ -478,6 +523,7 @@ def main():
                     help="also run a 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--no-ssr", action="store_true", help="skip the end-to-end SSR leg")
     ap.add_argument("--no-p4", action="store_true", help="skip the P4 matrix-core leg")
+    ap.add_argument("--no-probes", action="store_true", help="skip the copy / triad / FP64 rate probes")
     ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows"],
                     help="P2 tangent path: element matrices + node gather (default) or the fused row kernel")
     args = ap.parse_args()
@@ -488,6 +534,9 @@ def main():
     if args.config == "p4":
         print(json.dumps(run_p4(args)), flush=True)
         return
+    probes = None if args.no_probes else run_probes()
+    if probes:
+        _log(f"probes: {probes}")
     res = run_gpu(args)
     op = res["op"]
     kern = res["kernels"]
@@ -543,6 +592,7 @@ def main():
                                "measured in this run") if res["traffic"] is not None else None,
         },
         "cpu_baseline": cpu,
+        "probes": probes,
         "p4": p4,
         "ssr": ssr,
         "kernels": kern,

@@ -259,6 +259,14 @@ int ssx_spd_inverse_tiles(int64_t n, const double *A, int f64, void *work, void 
 int ssx_sym_tiles_gemv_f32(int64_t n, const float *tiles, const float *x, float *part, float *y, void *stream);
 int ssx_sym_tiles_gemv_f64(int64_t n, const double *tiles, const double *x, double *part, double *y, void *stream);
 
+/* ---- device probe (no reference counterpart: measurement only) ---- */
+
+/* Runs `blocks` workgroups of 256 threads, each lane `iters` rounds of independent FP64 dependency
+ * chains: mode 0 vector FMA, mode 1 v_mfma_f64_16x16x4 (four chains per wavefront).  out receives
+ * blocks*256 doubles (keeps the work alive); *flops (host) the FP64 operations launched, so
+ * flops / elapsed time is the rate this GPU attains. */
+int ssx_probe_fp64(int mode, int64_t blocks, int64_t iters, double *out, double *flops, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

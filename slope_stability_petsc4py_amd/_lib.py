@@ -57,6 +57,7 @@ _SIGS: dict[str, list] = {
     "ssx_galerkin_csr": [I64, P, P, P, I64, P, P, P, P, P, I64, F64, P, P, P],
     "ssx_element_matrices_p4": [P, P, P, I64, F64, P, P, P, P],
     "ssx_element_matrices_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P],
+    "ssx_probe_fp64": [I32, I64, I64, P, P, P],
     "ssx_gather_node_rows": [I32, I32, P, P, P, P, P, P, P, P, I64, I32, F64, P, P, P, P],
     "ssx_gemm_bf16": [I64, I64, I64, P, I64, I64, P, I64, I64, P, I64, P, I32, P, F32, P],
     "ssx_tanh_bwd": [I64, P, P, P, P, P],

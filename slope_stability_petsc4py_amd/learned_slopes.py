@@ -92,10 +92,14 @@ def collect(case_of, seeds, *, precision: str = "fp64", out_dir=None, log=print)
 
 
 def samples_from_runs(case_of, run_root, seeds):
-    """Training samples from runs written by `collect(out_dir=...)` (no continuation is run)."""
+    """Training samples from runs written by `collect(out_dir=...)` (no continuation is run); seeds
+    without a run are skipped."""
     from .runio import read_run, step_states
     samples = []
     for seed in seeds:
+        if not (Path(run_root) / f"seed_{seed}" / "petsc_run.npz").exists():
+            print(f"seed {seed}: no run in {run_root}, skipped", file=sys.stderr, flush=True)
+            continue
         run = read_run(Path(run_root) / f"seed_{seed}")
         case = case_of(seed)
         info = node_info(case.mesh, case.materials)

@@ -863,7 +863,14 @@ class PMultigrid:
 class DeflatedFGMRES:
     def __init__(self, preconditioner, *, tolerance: float = 1e-1, max_iterations: int = 100,
                  tolerance_deflation_basis: float = 1e-3, max_deflation_basis_vectors: int = 48,
-                 reorth_passes: int = 1):
+                 reorth_passes: int = 1, orthogonalization: str = "cgs2"):
+        """``orthogonalization``: "cgs2" (classical Gram-Schmidt plus ``reorth_passes`` passes, two
+        batched launches per pass) or "mgs" (modified Gram-Schmidt in the reference's order,
+        linear/deflated_fgmres.py:259-263: one dot and one update per basis vector, no
+        re-orthogonalisation)."""
+        if orthogonalization not in ("cgs2", "mgs"):
+            raise ValueError(f"unknown orthogonalization {orthogonalization!r}")
+        self.orthogonalization = orthogonalization
         self.pc = preconditioner
         self.tolerance = float(tolerance)
         self.max_iterations = int(max_iterations)
@@ -903,7 +910,8 @@ class DeflatedFGMRES:
     def copy(self):
         c = DeflatedFGMRES(self.pc, tolerance=self.tolerance, max_iterations=self.max_iterations,
                            tolerance_deflation_basis=self.tolerance_deflation_basis,
-                           max_deflation_basis_vectors=self.max_basis, reorth_passes=self.reorth_passes)
+                           max_deflation_basis_vectors=self.max_basis, reorth_passes=self.reorth_passes,
+                           orthogonalization=self.orthogonalization)
         c.deflation_basis = self.deflation_basis
         c.iteration_collector = self.iteration_collector
         return c
@@ -1019,12 +1027,17 @@ class DeflatedFGMRES:
                 _lib.call("ssx_gemv_cols", n, k, ptr(W), n, ptr(cbuf), -1.0, ptr(Z[j]), _st())
             u = V[j + 1]
             spmv(A, Z[j], u)
-            _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf), _st())
-            _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf), -1.0, ptr(u), _st())
-            for _ in range(self.reorth_passes):
-                _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf2), _st())
-                _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf2), -1.0, ptr(u), _st())
-                hbuf[: j + 1] += hbuf2[: j + 1]
+            if self.orthogonalization == "mgs":
+                for i in range(j + 1):
+                    _lib.call("ssx_dot_cols", n, 1, ptr(V[i]), n, ptr(u), ptr(scr.cols), ptr(hbuf[i:]), _st())
+                    _lib.call("ssx_gemv_cols", n, 1, ptr(V[i]), n, ptr(hbuf[i:]), -1.0, ptr(u), _st())
+            else:
+                _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf), _st())
+                _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf), -1.0, ptr(u), _st())
+                for _ in range(self.reorth_passes):
+                    _lib.call("ssx_dot_cols", n, j + 1, ptr(V), n, ptr(u), ptr(scr.cols), ptr(hbuf2), _st())
+                    _lib.call("ssx_gemv_cols", n, j + 1, ptr(V), n, ptr(hbuf2), -1.0, ptr(u), _st())
+                    hbuf[: j + 1] += hbuf2[: j + 1]
             # |u|^2 lands in hbuf[j+1]: the Hessenberg column and the norm come back in one read
             _lib.call("ssx_dot", n, ptr(u), ptr(u), ptr(scr.part), ptr(hbuf[j + 1:]), _st())
             col = hbuf[: j + 2].cpu().numpy()

@@ -13,6 +13,7 @@ read back offline, e.g. as training data of the learned predictor (`read_run`).
 from __future__ import annotations
 
 import json
+import os
 import platform
 import time
 from pathlib import Path
@@ -47,7 +48,11 @@ def write_run(out_dir, case, solver, res, *, analysis: str = "ssr", precision: s
     }
     if res.step_U:
         arrays["step_U"] = np.stack([_field(u, dim) for u in res.step_U])
-    np.savez_compressed(out / "petsc_run.npz", **arrays)
+    # written under a temporary name and renamed, so an interrupted writer leaves no partial file
+    tmp = out / "petsc_run.npz.part"
+    with open(tmp, "wb") as fh:
+        np.savez_compressed(fh, **arrays)
+    os.replace(tmp, out / "petsc_run.npz")
     p = case.params
     learned = sum(1 for r in res.step_info if r.get("predictor") == "learned")
     gated = sum(1 for r in res.step_info if r.get("predictor") == "learned_gated")

@@ -54,6 +54,29 @@ def test_fgmres_solves_elastic_system(gpu, elastic3d, use32, pc):
         assert its < 120, its
 
 
+def test_fgmres_mgs_order_matches_cgs2(gpu, elastic3d):
+    """The reference's modified Gram-Schmidt order (orthogonalization="mgs") and the default
+    CGS + one re-orthogonalisation pass build the same Krylov basis in exact arithmetic: the same
+    solution, iteration count and residual history to rounding."""
+    m, op, K, f = elastic3d
+    b = torch.from_numpy(f).to(gpu)
+    out = {}
+    for orth in ("cgs2", "mgs"):
+        prec = PMultigrid(m.elem, m.q_mask, use32=False)
+        solver = DeflatedFGMRES(prec, tolerance=1e-8, max_iterations=300, orthogonalization=orth)
+        solver.setup_preconditioner(K)
+        solver.A_orthogonalize(K)
+        x = solver.solve(K, b).cpu().numpy()
+        out[orth] = (x, solver.last_residual_history)
+    (x1, h1), (x2, h2) = out["cgs2"], out["mgs"]
+    assert abs(len(h1) - len(h2)) <= 1
+    k = min(len(h1), len(h2))
+    np.testing.assert_allclose(h2[:k], h1[:k], rtol=1e-6)
+    assert np.linalg.norm(x2 - x1) <= 1e-7 * np.linalg.norm(x1)
+    with pytest.raises(ValueError):
+        DeflatedFGMRES(None, orthogonalization="householder")
+
+
 def test_sparse_galerkin_matches_dense_on_the_p1_pattern(gpu, elastic3d):
     """P^T A P on the P1 pattern (ssx_galerkin_csr) against the dense product (ssx_galerkin_dense):
     equal at the pattern's positions, and the dense product vanishes (to rounding) outside them."""
@@ -354,3 +377,30 @@ def test_footing_config_size_forms_agree(gpu):
         w = min(ref[0][-1], out[form][0][-1])
         t_ref = np.interp(w, *ref)
         assert abs(np.interp(w, *out[form]) - t_ref) <= 1e-3 * t_ref, form
+
+
+def test_p4_layered_slope_fos_forms_agree(gpu):
+    """The heterogeneous P4 configuration to its factor of safety: the reference's L2 slope mesh
+    elevated to P4 (33,935 tets, 1.14M dofs) with the four layer materials, p-multigrid
+    P4 -> P2 -> P1.  The FP32-preconditioner form and the plain FP64 form reach the same FoS within
+    1e-3; at the last common omega the FP32 form's lambda is within 1e-3 of the FP64 form's."""
+    import time
+    case = cases.hetero3d_L2_layers()
+    assert case.mesh.elem_type == "P4" and case.mesh.n_elem == 33_935
+    out = {}
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        s = cases.make_solver(case, precision=prec)
+        assert s.op.pattern.n_rows > 1_100_000
+        res = cases.run_ssr(s)
+        out[prec] = res
+        print(f"case: P4 layered slope {prec}: FoS {res.fos:.6f} stop {res.stop_reason} steps "
+              f"{len(res.lambda_hist)} newton {res.newton_iterations} fgmres {res.linear_iterations} "
+              f"wall {res.wall_time:.1f}s total {time.perf_counter() - t0:.1f}s", flush=True)
+        del s
+        torch.cuda.empty_cache()
+    a, b = out["fp64"], out["fp32"]
+    assert a.stop_reason == "d_lambda_diff_scaled_min" and b.stop_reason == "d_lambda_diff_scaled_min"
+    assert abs(a.fos - b.fos) < 1e-3
+    w = min(a.omega_hist[-1], b.omega_hist[-1])
+    assert abs(np.interp(w, a.omega_hist, a.lambda_hist) - np.interp(w, b.omega_hist, b.lambda_hist)) < 1e-3
