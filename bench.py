@@ -552,6 +552,9 @@ def main():
         _log("P4 leg (matrix-core element matrices)...")
         r4 = run_p4(args)
         p4 = {k: r4[k] for k in ("value", "unit", "ms_per_step", "config", "roofline", "kernels")}
+        if probes:
+            # the same achieved rate against what this GPU sustains in the pure-MFMA probe
+            p4["roofline"]["frac_of_probe"] = round(p4["roofline"]["achieved"] / probes["fp64_mfma_TFLOP_s"], 4)
     ssr = None
     if not args.no_ssr:
         _log("end-to-end SSR leg (3D heterogeneous P2)...")

@@ -26,39 +26,39 @@ constexpr int NQ4 = 24;
 constexpr int KDIM = NQ4 * 6;  // 144
 constexpr int TILES = 7;        // ceil(105 / 16)
 
-// Column of B for local dof i = 3*node + d (gradients (gx, gy, gz) of that node), as the six
-// strain-row coefficients, branch-free (0/1 weights): no divergent branch selects an address or
-// an LDS offset anywhere in this file (hipcc for gfx950 has misplaced such selections under the
-// wrong execution mask).
-__device__ __forceinline__ void bcol(int d, double gx, double gy, double gz, double c[6]) {
-    const double c0 = d == 0 ? 1.0 : 0.0, c1 = d == 1 ? 1.0 : 0.0, c2 = d == 2 ? 1.0 : 0.0;
-    c[0] = c0 * gx;
-    c[1] = c1 * gy;
-    c[2] = c2 * gz;
-    c[3] = c0 * gy + c1 * gx;
-    c[4] = c1 * gz + c2 * gy;
-    c[5] = c0 * gz + c2 * gx;
-}
-
-// Entry s of a six-vector, branch-free.
-__device__ __forceinline__ double pick6(int s, const double c[6]) {
-    double v = 0.0;
-#pragma unroll
-    for (int r = 0; r < 6; ++r) v += (s == r ? 1.0 : 0.0) * c[r];
-    return v;
-}
+// No divergent branch selects an address or an LDS offset anywhere in this file (hipcc for gfx950
+// has misplaced such selections under the wrong execution mask): the sparsity of B is read from
+// bit tables into plain integer arithmetic, and zero entries are 0/1 weights on a valid load.
 
 // One workgroup of four wavefronts per element.  The Bm operand (w D_r B, 144 x 112 padded) is
 // formed once per element into LDS, in three chunks of KC = 48 k values (8 points), fragment-major:
 // sb[tj][ks][lane] is the value lane `lane` feeds v_mfma_f64_16x16x4 at k-step ks of tile column
 // tj (k = 4 ks + lane / 16, j = 16 tj + lane % 16), so each MFMA reads one contiguous 512-byte
-// fragment.  The A operand (B^T) costs a few VALU operations per k-step and tile row, and is formed
-// in registers.  The 28 upper tiles are split 7 per wavefront: wavefront w takes tile row w (tiles
+// fragment.  A column of B has three nonzeros, so a Bm value costs three FMAs; an A value (an
+// entry of B) is one gradient read or zero, formed in registers per k-step and tile row.  The 28 upper tiles are split 7 per wavefront: wavefront w takes tile row w (tiles
 // w..6) and tile row 7 - w (tiles 7 - w..6), so its slot t is (w, w + t) for t < 7 - w, else
 // (7 - w, t).  LDS: 20,160 B gradients + 6,912 B w D + 43,008 B fragments = 70,080 B, so two
 // elements (eight wavefronts) share a CU and one element's operand formation overlaps the other's
 // MFMAs.
 constexpr int NPAD = 16 * TILES;   // 112
+
+// Sparsity of B (strain rows e11 e22 e33 g12 g23 g13, displacement component d of a node):
+// kStrainC, 2 bits at 3 s + d: the gradient component (0 x, 1 y, 2 z) in row s, 3 = zero.
+constexpr uint64_t strain_c_table() {
+    const int c[6][3] = {{0, 3, 3}, {3, 1, 3}, {3, 3, 2}, {1, 0, 3}, {3, 2, 1}, {2, 3, 0}};
+    uint64_t v = 0;
+    for (int s = 0; s < 6; ++s)
+        for (int d = 0; d < 3; ++d) v |= (uint64_t)c[s][d] << (2 * (3 * s + d));
+    return v;
+}
+constexpr uint64_t kStrainC = strain_c_table();
+// Per component d (12 bits at 12 d): the three nonzero rows t1 < t2 < t3 of a column of B
+// (kRowT, 4 bits each) and the gradient components c1, c2, c3 in them (kGradC).
+//   d = 0: rows 0, 3, 5 <- x, y, z;  d = 1: rows 1, 3, 4 <- y, x, z;  d = 2: rows 2, 4, 5 <- z, y, x
+constexpr uint64_t kRowT = (0x0 | 0x3 << 4 | 0x5 << 8) | (uint64_t)(0x1 | 0x3 << 4 | 0x4 << 8) << 12 |
+                           (uint64_t)(0x2 | 0x4 << 4 | 0x5 << 8) << 24;
+constexpr uint64_t kGradC = (0x0 | 0x1 << 4 | 0x2 << 8) | (uint64_t)(0x1 | 0x0 << 4 | 0x2 << 8) << 12 |
+                            (uint64_t)(0x2 | 0x1 << 4 | 0x0 << 8) << 24;
 constexpr int KC = 48;             // k values per LDS chunk (8 points x 6 strains)
 constexpr int KSC = KC / 4;        // MFMA k-steps per chunk
 constexpr int NCHUNK = KDIM / KC;  // 3
@@ -108,37 +108,51 @@ __global__ void __launch_bounds__(256, 2) k_ke_p4_mfma(const double *__restrict_
     d4 acc[TILES];
 #pragma unroll
     for (int t = 0; t < TILES; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    const double *sgf = &sg[0][0][0];
+    const double *sdf = &sd[0][0];
     double *sbf = &sb[0][0][0];
+    // Bm formation: thread f < 2 * NPAD forms column j = f % NPAD for the 24 k values (4 points x 6
+    // strains) of half h = f / NPAD of each chunk; the last 32 threads do not form.  Column j (node
+    // fn, component fd) of B_q has three nonzeros: rows t1 < t2 < t3 holding gradient components
+    // c1, c2, c3 (kRowT / kGradC), so per point three gradient reads serve the six strains, and
+    // Bm[k][j] = sum_i wD_q[s][t_i] g_{c_i}.
+    constexpr int GS = NQ4 * NP4;                 // stride of a gradient component in sg
+    const bool former = tid < 2 * NPAD;
+    const int fj = tid % NPAD, fh = tid / NPAD;
+    const double fv = fj < NLD4 ? 1.0 : 0.0;
+    const int fjc = fj < NLD4 ? fj : NLD4 - 1;
+    const int fn = fjc / 3, fd = fjc - 3 * (fjc / 3);
+    const int ftd = (int)((kRowT >> (12 * fd)) & 0xfff), ftc = (int)((kGradC >> (12 * fd)) & 0xfff);
+    const int o1 = 6 * (ftd & 15), o2 = 6 * ((ftd >> 4) & 15), o3 = 6 * (ftd >> 8);
+    const int g1 = (ftc & 15) * GS + fn, g2 = ((ftc >> 4) & 15) * GS + fn, g3 = (ftc >> 8) * GS + fn;
+    double *fout = sbf + (fj >> 4) * (KSC * 64) + (fj & 15);   // + ksl * 64 + kq * 16
     for (int ch = 0; ch < NCHUNK; ++ch) {
         __syncthreads();   // staging done (first chunk) / the previous chunk's fragments are read
-        // Bm fragments of this chunk: element f = (tj * KSC + ksl) * 64 + fl
-        for (int f = tid; f < TILES * KSC * 64; f += 256) {
-            const int fl = f & 63, tk = f >> 6;
-            const int tj = tk / KSC, ksl = tk - (tk / KSC) * KSC;
-            const int k = ch * KC + 4 * ksl + (fl >> 4);
-            const int q = k / 6, s = k - (k / 6) * 6;
-            const int jb = 16 * tj + (fl & 15);
-            const double vb = jb < NLD4 ? 1.0 : 0.0;
-            const int jbc = jb < NLD4 ? jb : NLD4 - 1;
-            const int nb = jbc / 3, db = jbc % 3;
-            double cb[6];
-            bcol(db, sg[0][q][nb], sg[1][q][nb], sg[2][q][nb], cb);
-            // row s of (wD) B: sum_t wD[s][t] B[t][jb], D column-major (D[s + 6 t])
-            const double *D = sd[q];
-            double b = D[s] * cb[0];
+        if (former) {
 #pragma unroll
-            for (int t = 1; t < 6; ++t) b += D[s + 6 * t] * cb[t];
-            sbf[f] = vb * b;
+            for (int qi = 0; qi < 4; ++qi) {
+                const int q = ch * (KC / 6) + fh * 4 + qi;
+                const double *gq = sgf + q * NP4;
+                const double G1 = gq[g1], G2 = gq[g2], G3 = gq[g3];
+                const double *D = sdf + q * 36;
+#pragma unroll
+                for (int s6 = 0; s6 < 6; ++s6) {
+                    const double b = D[s6 + o1] * G1 + D[s6 + o2] * G2 + D[s6 + o3] * G3;
+                    const int kl = 6 * qi + s6;              // k within the half chunk (24 = 6 k-steps)
+                    fout[(fh * 6 + (kl >> 2)) * 64 + (kl & 3) * 16] = fv * b;
+                }
+            }
         }
         __syncthreads();
+#pragma unroll 4
         for (int ksl = 0; ksl < KSC; ++ksl) {
             const int k = ch * KC + 4 * ksl + kq;
-            const int q = k / 6, s = k - (k / 6) * 6;
-            double ca[6];
-            bcol(daA, sg[0][q][naA], sg[1][q][naA], sg[2][q][naA], ca);
-            const double aA = vaA * pick6(s, ca);
-            bcol(daB, sg[0][q][naB], sg[1][q][naB], sg[2][q][naB], ca);
-            const double aB = vaB * pick6(s, ca);
+            const int q = k / 6, s = k - 6 * (k / 6);
+            // A value B_q[s][i]: gradient component kStrainC(s, d) of node n, or zero
+            const int cA = (int)((kStrainC >> (2 * (3 * s + daA))) & 3);
+            const int cB = (int)((kStrainC >> (2 * (3 * s + daB))) & 3);
+            const double aA = (cA < 3 ? vaA : 0.0) * sgf[(cA < 3 ? cA : 0) * (NQ4 * NP4) + q * NP4 + naA];
+            const double aB = (cB < 3 ? vaB : 0.0) * sgf[(cB < 3 ? cB : 0) * (NQ4 * NP4) + q * NP4 + naB];
 #pragma unroll
             for (int t = 0; t < TILES; ++t) {
                 const bool first = t < nA;                // wavefront-uniform
