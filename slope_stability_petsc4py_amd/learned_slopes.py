@@ -142,6 +142,7 @@ def main(argv=None):
     ap.add_argument("--train", type=int, default=200)
     ap.add_argument("--seed-start", type=int, default=0,
                     help="first training seed (a shard of the training set: seeds seed-start .. seed-start+train-1)")
+    ap.add_argument("--seeds", default=None, help="explicit training seeds, comma separated (instead of the range)")
     ap.add_argument("--init", default=None,
                     help="continue training from these weights (training over shards, one call per shard)")
     ap.add_argument("--test", type=int, default=20)
@@ -164,13 +165,14 @@ def main(argv=None):
         test_seeds = [-1]
     t0 = time.perf_counter()
     runs = []
+    train_seeds = ([int(x) for x in a.seeds.split(",") if x] if a.seeds else
+                   list(range(a.seed_start, a.seed_start + a.train)))
     if a.load:
         samples = []
     elif a.from_runs:
-        samples = samples_from_runs(case_of, a.from_runs, range(a.seed_start, a.seed_start + a.train))
+        samples = samples_from_runs(case_of, a.from_runs, train_seeds)
     else:
-        samples, runs = collect(case_of, range(a.seed_start, a.seed_start + a.train), precision=a.precision,
-                                out_dir=a.out, log=log)
+        samples, runs = collect(case_of, train_seeds, precision=a.precision, out_dir=a.out, log=log)
     t_data = time.perf_counter() - t0
     t1 = time.perf_counter()
     if a.load:
@@ -183,7 +185,8 @@ def main(argv=None):
     if a.save:
         net.save(a.save)
     evals = evaluate(net, case_of, test_seeds, precision=a.precision, log=log) if a.test > 0 else []
-    out = {"case": a.case, "h": a.h, "precision": a.precision, "seed_start": a.seed_start, "train_seeds": a.train, "test_seeds": len(evals), "samples": len(samples),
+    out = {"case": a.case, "h": a.h, "precision": a.precision, "train_seeds": train_seeds, "test_seeds": len(evals),
+           "samples": len(samples),
            "epochs": a.epochs, "loss_first": losses[0] if losses else None,
            "loss_last": losses[-1] if losses else None, "data_gen_s": t_data, "train_s": t_train,
            "train_runs": runs, "eval": evals, **(summary(evals) if evals else {})}
