@@ -230,14 +230,17 @@ int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const int32_t *ind
                        double *Ac, void *work, void *stream);
 
 /* Sparse Galerkin coarse operator onto a given coarse CSR pattern (c_indptr, c_indices sorted per
- * row, c_nnz entries): c_val (caller-zeroed) = P^T A P at the positions the pattern holds, products
- * outside it dropped (for nested P1-in-P2 spaces they sum to zero).  Fixed-point sums as in
- * ssx_galerkin_dense (bitwise repeatable).  Replaces the same PETSc MatPtAP (linear/solver.py:510-513)
- * for coarse levels too large for a dense matrix. */
+ * row, n_c rows, c_nnz entries, rows of at most max_c_row <= 1024 entries): c_val = P^T A P at the
+ * positions the pattern holds, products outside it dropped (for nested P1-in-P2 spaces they sum to
+ * zero).  pt_ptr/pt_idx/pt_w is P^T in CSR (n_c rows): each coarse row is gathered by one
+ * wavefront, every entry written once.  Fixed-point sums as in ssx_galerkin_dense (bitwise
+ * repeatable, the same bits as the scatter form).  Replaces the same PETSc MatPtAP
+ * (linear/solver.py:510-513) for coarse levels too large for a dense matrix. */
 int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val, int64_t nnz,
-                     const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, const int32_t *c_indptr,
-                     const int32_t *c_indices, int64_t c_nnz, double w_bound, double *c_val, void *work,
-                     void *stream);
+                     const int32_t *p_ptr, const int32_t *p_idx, const double *p_w, const int32_t *pt_ptr,
+                     const int32_t *pt_idx, const double *pt_w, int64_t n_c, const int32_t *c_indptr,
+                     const int32_t *c_indices, int64_t c_nnz, int max_c_row, double w_bound, double *c_val,
+                     void *work, void *stream);
 
 /* Coarse solve y = Ainv x of the p-multigrid with the explicit inverse of the symmetric coarse
  * operator kept as the upper triangle of 64 x 64 tiles (half the bytes of a dense product per

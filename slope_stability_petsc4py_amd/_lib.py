@@ -54,7 +54,7 @@ _SIGS: dict[str, list] = {
     "ssx_inv_diag_f32": [I64, I64, P, P, P, P, P],
     "ssx_inv_diag_f64": [I64, I64, P, P, P, P, P],
     "ssx_galerkin_dense": [I64, P, P, P, I64, P, P, P, I64, F64, P, P, P],
-    "ssx_galerkin_csr": [I64, P, P, P, I64, P, P, P, P, P, I64, F64, P, P, P],
+    "ssx_galerkin_csr": [I64, P, P, P, I64, P, P, P, P, P, P, I64, P, P, I64, I32, F64, P, P, P],
     "ssx_element_matrices_p4": [P, P, P, I64, F64, P, P, P, P],
     "ssx_element_matrices_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P],
     "ssx_probe_fp64": [I32, I64, I64, P, P, P],

@@ -144,7 +144,7 @@ __global__ void __launch_bounds__(256, 2) k_ke_p4_mfma(const double *__restrict_
             }
         }
         __syncthreads();
-#pragma unroll 4
+#pragma unroll
         for (int ksl = 0; ksl < KSC; ++ksl) {
             const int k = ch * KC + 4 * ksl + kq;
             const int q = k / 6, s = k - 6 * (k / 6);

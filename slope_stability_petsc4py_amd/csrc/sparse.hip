@@ -442,11 +442,12 @@ int ssx_inv_diag_f64(int64_t n_rows, int64_t col0, const int32_t *indptr, const 
 
 // ---------------------------------------------------------------------------------------------
 // Galerkin coarse operators of the p-multigrid, Ac = P^T A P (replaces the PETSc MatPtAP of
-// linear/solver.py:510-513 feeding the coarse solve).  One lane per nonzero a_ij of the fine
-// operator scatters its products p_iI a_ij p_jJ (P has a few entries per fine row: p_ptr/p_idx/
-// p_w, CSR) into the coarse matrix.
+// linear/solver.py:510-513 feeding the coarse solve).  Dense form: one lane per nonzero a_ij of
+// the fine operator scatters its products p_iI a_ij p_jJ (P has a few entries per fine row:
+// p_ptr/p_idx/p_w, CSR) into the coarse matrix.  CSR form: one wavefront per coarse row gathers
+// them (below).
 //
-// The scatter sums in 64-bit fixed point: each product is rounded to an integer multiple of 2^-f
+// Both sum in 64-bit fixed point: each product is rounded to an integer multiple of 2^-f
 // and added with an integer atomic.  Integer addition is associative, so every coarse entry is the
 // same bit pattern whatever order the atomics land in (FP64 atomics give a different rounding at
 // every run).  The scale comes from max|a| (an atomicMax on the IEEE bits of |a|, which order the
@@ -508,39 +509,58 @@ __global__ void k_galerkin_dense(int64_t n_rows, const int32_t *__restrict__ ind
     }
 }
 
-// Coarse operator onto a given coarse CSR pattern (columns sorted within a row; binary search per
-// product).  Products outside the pattern are dropped: for nested spaces (P1 inside P2 on one
-// mesh, P the interpolation) P^T A P is the coarse stiffness matrix, whose sparsity is the coarse
-// node adjacency, so those products sum to zero up to rounding.
-__global__ void k_galerkin_csr(int64_t n_rows, const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
-                               const double *__restrict__ val, const int32_t *__restrict__ p_ptr,
-                               const int32_t *__restrict__ p_idx, const double *__restrict__ p_w,
-                               const int32_t *__restrict__ c_indptr, const int32_t *__restrict__ c_indices,
-                               const u64 *__restrict__ amax, double w_bound, u64 *__restrict__ c_val) {
-    const int64_t row = (int64_t)blockIdx.x * blockDim.y + threadIdx.y;
-    if (row >= n_rows) return;
-    const int32_t pi0 = p_ptr[row], pi1 = p_ptr[row + 1];
-    if (pi0 == pi1) return;
+// Coarse operator onto a given coarse CSR pattern (columns sorted within a row), gathered per coarse
+// row: one wavefront per row I takes the fine rows i with P[i][I] != 0 (the transpose of P,
+// pt_ptr/pt_idx/pt_w), and for every nonzero a_ij of those rows and every P[j][J] adds
+// P[i][I] a_ij P[j][J] to entry (I, J), found by binary search in the row's column list staged in
+// LDS, with a 64-bit integer atomic in LDS.  The products and their fixed-point rounding are the
+// scatter form's, the sums are exact integers, and each coarse entry is written once: no global
+// atomics, the same bits at every call.  Products outside the pattern are dropped: for nested
+// spaces (P1 inside P2 on one mesh, P the interpolation) P^T A P is the coarse stiffness matrix,
+// whose sparsity is the coarse node adjacency, so those products sum to zero up to rounding.
+__global__ void __launch_bounds__(256) k_galerkin_rows(
+    int64_t n_c, const int32_t *__restrict__ indptr, const int32_t *__restrict__ indices,
+    const double *__restrict__ val, const int32_t *__restrict__ p_ptr, const int32_t *__restrict__ p_idx,
+    const double *__restrict__ p_w, const int32_t *__restrict__ pt_ptr, const int32_t *__restrict__ pt_idx,
+    const double *__restrict__ pt_w, const int32_t *__restrict__ c_indptr, const int32_t *__restrict__ c_indices,
+    int max_row, const u64 *__restrict__ amax, double w_bound, double *__restrict__ c_val) {
+    extern __shared__ u64 smem[];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    u64 *acc = smem + (int64_t)wv * max_row;
+    int32_t *cols = reinterpret_cast<int32_t *>(smem + 4 * (int64_t)max_row) + (int64_t)wv * max_row;
+    const int64_t I = (int64_t)blockIdx.x * 4 + wv;
+    const bool live = I < n_c;
+    const int32_t r0 = live ? c_indptr[I] : 0;
+    const int32_t len = live ? c_indptr[I + 1] - r0 : 0;
+    const int32_t L = len < max_row ? len : max_row;   // max_row is the pattern's longest row
+    for (int p = lane; p < L; p += 64) {
+        cols[p] = c_indices[r0 + p];
+        acc[p] = 0ull;
+    }
+    __syncthreads();
     const double scale = fixed_scale(amax, w_bound);
-    for (int32_t k = indptr[row] + threadIdx.x; k < indptr[row + 1]; k += blockDim.x) {
-        const int32_t col = indices[k];
-        const double a = val[k];
-        const int32_t pj0 = p_ptr[col], pj1 = p_ptr[col + 1];
-        for (int32_t s = pi0; s < pi1; ++s) {
-            const double ai = a * p_w[s];
-            const int32_t I = p_idx[s];
-            const int32_t r0 = c_indptr[I], r1 = c_indptr[I + 1];
-            for (int32_t t = pj0; t < pj1; ++t) {
-                const int32_t J = p_idx[t];
-                int32_t lo = r0, hi = r1;     // first position with c_indices[pos] >= J
-                while (lo < hi) {
-                    const int32_t mid = lo + ((hi - lo) >> 1);
-                    if (c_indices[mid] < J) lo = mid + 1; else hi = mid;
+    if (live) {
+        for (int32_t c = pt_ptr[I]; c < pt_ptr[I + 1]; ++c) {
+            const int32_t i = pt_idx[c];
+            const double wI = pt_w[c];
+            for (int32_t k = indptr[i] + lane; k < indptr[i + 1]; k += 64) {
+                const int32_t j = indices[k];
+                const double ai = val[k] * wI;
+                for (int32_t t = p_ptr[j]; t < p_ptr[j + 1]; ++t) {
+                    const int32_t J = p_idx[t];
+                    int32_t lo = 0, hi = L;     // first position with cols[pos] >= J
+                    while (lo < hi) {
+                        const int32_t mid = lo + ((hi - lo) >> 1);
+                        if (cols[mid] < J) lo = mid + 1; else hi = mid;
+                    }
+                    if (lo < L && cols[lo] == J) atomicAdd(&acc[lo], (u64)__double2ll_rn(ai * p_w[t] * scale));
                 }
-                if (lo < r1 && c_indices[lo] == J) add_fixed(&c_val[lo], ai * p_w[t], scale);
             }
         }
     }
+    __syncthreads();
+    const double inv = 1.0 / scale;   // a power of two: exact
+    for (int p = lane; p < L; p += 64) c_val[r0 + p] = (double)(long long)acc[p] * inv;
 }
 
 // In place: fixed-point sums -> FP64.
@@ -581,21 +601,24 @@ extern "C" int ssx_galerkin_dense(int64_t n_rows, const int32_t *indptr, const i
 
 extern "C" int ssx_galerkin_csr(int64_t n_rows, const int32_t *indptr, const int32_t *indices, const double *val,
                                 int64_t nnz, const int32_t *p_ptr, const int32_t *p_idx, const double *p_w,
-                                const int32_t *c_indptr, const int32_t *c_indices, int64_t c_nnz, double w_bound,
-                                double *c_val, void *work, void *stream) {
-    SSX_REQUIRE(n_rows >= 0 && c_nnz >= 0 && nnz >= 0, "ssx_galerkin_csr: bad sizes");
+                                const int32_t *pt_ptr, const int32_t *pt_idx, const double *pt_w, int64_t n_c,
+                                const int32_t *c_indptr, const int32_t *c_indices, int64_t c_nnz, int max_c_row,
+                                double w_bound, double *c_val, void *work, void *stream) {
+    SSX_REQUIRE(n_rows >= 0 && n_c >= 0 && c_nnz >= 0 && nnz >= 0, "ssx_galerkin_csr: bad sizes");
     SSX_REQUIRE(w_bound > 0.0, "ssx_galerkin_csr: w_bound must be positive");
-    if (n_rows == 0 || c_nnz == 0) return SSX_OK;
-    SSX_REQUIRE(indptr && indices && val && p_ptr && p_idx && p_w && c_indptr && c_indices && c_val && work,
+    SSX_REQUIRE(max_c_row >= 0 && max_c_row <= 1024, "ssx_galerkin_csr: coarse rows of %d entries (at most 1024)",
+                max_c_row);
+    if (n_rows == 0 || n_c == 0 || c_nnz == 0) return SSX_OK;
+    SSX_REQUIRE(indptr && indices && val && p_ptr && p_idx && p_w && pt_ptr && pt_idx && pt_w && c_indptr &&
+                    c_indices && c_val && work,
                 "ssx_galerkin_csr: null pointer");
     hipStream_t st = ssx::as_stream(stream);
     u64 *amax = reinterpret_cast<u64 *>(work);
     galerkin_prologue(nnz, val, amax, st);
-    dim3 block(16, 16);
-    hipLaunchKernelGGL(k_galerkin_csr, dim3(ssx::grid_for(n_rows, 16)), block, 0, st, n_rows, indptr, indices, val,
-                       p_ptr, p_idx, p_w, c_indptr, c_indices, amax, w_bound, reinterpret_cast<u64 *>(c_val));
-    hipLaunchKernelGGL(k_fixed_to_f64, dim3(ssx::grid_for(c_nnz < 4096 * 256 ? c_nnz : 4096 * 256, 256)), dim3(256),
-                       0, st, c_nnz, reinterpret_cast<u64 *>(c_val), amax, w_bound);
+    const int L = max_c_row > 0 ? max_c_row : 1;
+    const size_t lds = (size_t)4 * L * (sizeof(u64) + sizeof(int32_t));
+    hipLaunchKernelGGL(k_galerkin_rows, dim3(ssx::grid_for(n_c, 4)), dim3(256), lds, st, n_c, indptr, indices, val,
+                       p_ptr, p_idx, p_w, pt_ptr, pt_idx, pt_w, c_indptr, c_indices, L, amax, w_bound, c_val);
     SSX_CHECK_LAUNCH("ssx_galerkin_csr");
     return SSX_OK;
 }

@@ -134,9 +134,10 @@ class DevicePattern:
             self.ns_elem = to_dev(ns.elem)
             self.ns_lnode = to_dev(ns.lnode)
             self.ns_off = to_dev(ns.off)
-            # one gather launch per row-length class when the rows are long (P4: a vertex row holds
-            # several times the columns of a face or interior node's row)
-            groups = ns.buckets() if ns.max_len > 192 else []
+            # one gather launch per row-length class when rows run long (beyond 640 entries the LDS
+            # of the longest row, 15 KB, would hold every workgroup to a few per CU; P4 vertex rows
+            # reach 2,457 entries against 105 for interior nodes); shorter meshes keep one launch
+            groups = ns.buckets() if ns.max_len > 640 else []
             self.ns_groups = [(to_dev(ids), ml) for ids, ml in groups] if len(groups) > 1 else []
         self._ke = None
         self.force_gather = "nodes"     # "nodes" (node slots, 3D) or "rows" (row slots)

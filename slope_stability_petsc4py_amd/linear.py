@@ -675,6 +675,7 @@ class PMultigrid:
                                 torch.zeros(cp.nnz, dtype=torch.float64, device=dev),
                                 torch.zeros(cp.nnz, dtype=torch.float32, device=dev), spmv_lanes(cp.indptr))
             self.kc_cdiag = torch.from_numpy(cp.constrained_diag_positions.astype(np.int64)).to(dev)
+            self.kc_max_row = int(np.diff(cp.indptr).max()) if cp.indptr.size > 1 else 0
             t_ptr, t_idx, t_w, n_t = self._t_host
             T = sp.csr_matrix((t_w, t_idx, t_ptr), shape=(self.n_c, n_t))
             self.T = DeviceCSR(self.n_c, *mk(T), None, spmv_lanes(T.indptr))
@@ -725,9 +726,9 @@ class PMultigrid:
             # the sparse P1 level follows the current operator at every setup
             t0 = perf_counter()
             Kc = self.Kc
-            Kc.values.zero_()
             _lib.call("ssx_galerkin_csr", A.n_rows, ptr(A.indptr), ptr(A.indices), ptr(A.values), A.nnz,
-                      ptr(self.p_ptr), ptr(self.p_idx), ptr(self.p_w), ptr(Kc.indptr), ptr(Kc.indices), Kc.nnz,
+                      ptr(self.p_ptr), ptr(self.p_idx), ptr(self.p_w), ptr(self.PT.indptr), ptr(self.PT.indices),
+                      ptr(self.PT.values), self.n_c, ptr(Kc.indptr), ptr(Kc.indices), Kc.nnz, self.kc_max_row,
                       self._w_bound(A, "p"), ptr(Kc.values), ptr(self._gwork), _st())
             Kc.values[self.kc_cdiag] = 1.0
             _lib.call("ssx_cast_f64_f32", Kc.nnz, ptr(Kc.values), ptr(Kc.values32), _st())

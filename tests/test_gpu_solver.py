@@ -127,10 +127,11 @@ def test_galerkin_bitwise_repeatable_and_matches_scipy(gpu, elastic3d, kind):
                       stream_ptr())
             got = out.cpu().numpy().T
         else:
-            out = torch.zeros(prec.Kc.nnz, dtype=torch.float64, device=gpu)
+            out = torch.full((prec.Kc.nnz,), np.nan, dtype=torch.float64, device=gpu)
             _lib.call("ssx_galerkin_csr", K.n_rows, ptr(K.indptr), ptr(K.indices), ptr(K.values), K.nnz,
-                      ptr(prec.p_ptr), ptr(prec.p_idx), ptr(prec.p_w), ptr(prec.Kc.indptr), ptr(prec.Kc.indices),
-                      prec.Kc.nnz, wb, ptr(out), ptr(prec._gwork), stream_ptr())
+                      ptr(prec.p_ptr), ptr(prec.p_idx), ptr(prec.p_w), ptr(prec.PT.indptr), ptr(prec.PT.indices),
+                      ptr(prec.PT.values), prec.n_c, ptr(prec.Kc.indptr), ptr(prec.Kc.indices), prec.Kc.nnz,
+                      prec.kc_max_row, wb, ptr(out), ptr(prec._gwork), stream_ptr())
             cp = prec.c_pat
             got = sps.csr_matrix((out.cpu().numpy(), cp.indices, cp.indptr), shape=exp.shape).toarray()
             on = sps.csr_matrix((np.ones(cp.indices.size), cp.indices, cp.indptr), shape=exp.shape).toarray() != 0
