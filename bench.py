@@ -397,17 +397,12 @@ def run_ssr2d_leg(log_progress=True) -> dict:
     f_int[1] = -case.materials[0].gamma_sat
     f = fem.volume_load(geom, m.elem, m.n_nodes, f_int)
     f[~m.q_mask.astype(bool)] = 0.0
-    try:
-        from threadpoolctl import threadpool_limits
-        limiter = threadpool_limits(1)
-    except Exception:
-        limiter = None
-    r = ssr2d.ssr(prob, f.T.reshape(-1).copy(), lambda_init=p.lambda_init, d_lambda_init=p.d_lambda_init,
-                  d_lambda_min=p.d_lambda_min, d_lambda_diff_scaled_min=p.d_lambda_diff_scaled_min,
-                  step_max=p.step_max, omega_max_stop=p.omega_max_stop, it_newt_max=p.it_newt_max,
-                  it_damp_max=p.it_damp_max, tol=p.tol, r_min=p.r_min)
-    if limiter is not None:
-        limiter.unregister()
+    from threadpoolctl import threadpool_limits
+    with threadpool_limits(1):       # one host core: the BLAS behind NumPy/scipy limited to one thread
+        r = ssr2d.ssr(prob, f.T.reshape(-1).copy(), lambda_init=p.lambda_init, d_lambda_init=p.d_lambda_init,
+                      d_lambda_min=p.d_lambda_min, d_lambda_diff_scaled_min=p.d_lambda_diff_scaled_min,
+                      step_max=p.step_max, omega_max_stop=p.omega_max_stop, it_newt_max=p.it_newt_max,
+                      it_damp_max=p.it_damp_max, tol=p.tol, r_min=p.r_min)
     cpu = {"value": round(r["wall"], 3), "unit": "s to FoS", "cores": 1, "kind": "port",
            "fos": r["fos"], "fos_abs_err": abs(r["fos"] - fos_ref), "newton": r["newton"],
            "sample": "the whole 2D SSR run (oracle/ssr2d.py, NumPy/scipy, direct solves, BLAS limited to one "

@@ -11,6 +11,9 @@ Contents
                       compiled from their sources (oracle/Makefile `ref`); None where not built.
   * NumPy restatements below: Davis reduction, 2D plane-strain return mapping, brute-force strain
     operator B, brute-force CSR pattern, dense element tangent.
+  * ``oracle.ssr2d`` -- the plain 2D SSR end to end on the host (vectorised return mapping, sparse
+    B^T D B, Newton with damping, the SSR continuation; SuperLU solves): the checker of the 2D SSR
+    and bench.py's CPU baseline of that leg.
 
 Pinning summary (what checks each part beyond retyping):
   davis_reduction   closed forms: for psi = 0 Davis B gives tan(phi_l) = sin(phi_1), phi_1 =
@@ -25,6 +28,9 @@ Pinning summary (what checks each part beyond retyping):
   spmv              scipy CSR product (tests/test_oracle_mc2d.py).
   strain/force      brute-force B from the reference's strain-operator rule (fem/assembly.py:203-230).
   pattern           set-based restatement of fem/distributed_tangent.py:440-478 and :369-437.
+  ssr2d             its return mapping equals mc2d entry for entry; its sparse tangent is the
+                    central-difference derivative of its force; its SSR run reproduces the
+                    reference's committed 2D lambda history (tests/test_oracle_ssr2d.py).
 """
 
 from __future__ import annotations
