@@ -288,6 +288,19 @@ def cpu_baseline(op, E_host, ab, budget_s: float):
         if time.perf_counter() - t0 > budget_s or reps >= 5:
             break
     per = (t_const + t_force + t_tan) / reps
+    # the tangent rows once more on one thread, for comparison with the reference's single-thread
+    # micro-benchmark figure (the OpenMP runtime the reference kernels were built against)
+    t_tan1 = None
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        a = time.perf_counter()
+        oracle.ref_tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+        t_tan1 = time.perf_counter() - a
+        gomp.omp_set_num_threads(threads)
+    except OSError:
+        pass
     step_bytes = ab["constitutive"] + ab["force"] + ab["tangent"]
     return {
         "value": round(step_bytes / per / 1e9, 2),
@@ -298,6 +311,7 @@ def cpu_baseline(op, E_host, ab, budget_s: float):
                   f"tangent rows (reference C, -O3 -fopenmp); s/step {per:.3f} "
                   f"(const {t_const / reps:.3f}, force {t_force / reps:.3f}, tangent {t_tan / reps:.3f})",
         "s_per_step": round(per, 4),
+        "tangent_rows_1_thread_s": None if t_tan1 is None else round(t_tan1, 3),
     }
 
 
