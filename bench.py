@@ -66,20 +66,22 @@ def build_workload(h: float, nz: int, n_unique: int, seed: int):
     return m, op, E, br
 
 
-def algorithmic_bytes(op, fold: bool = True) -> dict:
-    """Per-launch algorithmic HBM bytes of each kernel of the step."""
+def algorithmic_bytes(op, fold: bool = True, branch: bool = False) -> dict:
+    """Per-launch algorithmic HBM bytes of each kernel of the step (`branch`: the return mapping
+    writes its int8 branch id per point and the tangent reads it)."""
     n = op.n_int
     pat = op.pattern
     nnz = pat.host.nnz
     n_p = pat.n_p
+    nb = 1 if branch else 0
     return {
-        # E 48 + (c_bar, sin_phi, shear, lame) 32 in; S 48 + DS 288 out
-        "constitutive": n * (48 + 32 + 48 + 288),
+        # E 48 + (c_bar, sin_phi, shear, lame) 32 in; S 48 + DS 288 (+ branch id) out
+        "constitutive": n * (48 + 32 + 48 + 288 + nb),
         # S 48 + dphi 3*n_p*8 + w 8 per point in; fe out; then fe in + F out
         "force": n * (48 + 24 * n_p + 8) + pat.n_elem * pat.nld * 8 * 2 + pat.n_rows * 8,
         # DS 288 + dphi 3*n_p*8 + w 8 (+ shear, lame 16 when K_r is folded into the point tangents,
         # else the assembled K_el nnz*8) in; K_r nnz*8 + FP32 copy nnz*4 out
-        "tangent": n * (288 + 24 * n_p + 8) + nnz * (8 + 4) + (n * 16 if fold else nnz * 8),
+        "tangent": n * (288 + 24 * n_p + 8 + nb) + nnz * (8 + 4) + (n * 16 if fold else nnz * 8),
         # dphi + u gather (3*n_p*8 per element, counted once) + E out
         "strain": n * (24 * n_p + 48) + pat.n_elem * pat.nld * 8,
         # CSR FP64: values 8 + index 4 per nnz, x 8 (once) + y 8 + indptr 4 per row
@@ -103,9 +105,12 @@ def run_gpu(args):
     st = stream_ptr()
     r = 1e-4
 
+    # branch ids of the return mapping go with DS into the tangent (elastic points in closed form)
+    br = op._branch if args.tangent == "nodes" else None
+
     def k_const():
         _lib.call("ssx_constitutive_3d", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear), ptr(op.lame),
-                  op.n_int, ptr(S), ptr(DS), None, None, st)
+                  op.n_int, ptr(S), ptr(DS), None, ptr(br), st)
 
     def k_force():
         pat.force(S, fe, F)
@@ -114,7 +119,7 @@ def run_gpu(args):
     el = None if mat is not None else op.elastic_values
 
     def k_tangent():
-        pat.tangent(DS, K.values, K.values32, r=r, elastic=el, mat=mat)
+        pat.tangent(DS, K.values, K.values32, r=r, elastic=el, mat=mat, branch=br if mat is not None else None)
 
     u = torch.from_numpy(np.random.default_rng(1).normal(0, 1e-3, pat.n_rows)).to(dev)
     E_tmp = torch.empty_like(op._E)
@@ -179,7 +184,7 @@ def run_gpu(args):
             rows.index_select(0, idx)
         torch.cuda.synchronize()
         del src, dst, rows, idx
-    ab = algorithmic_bytes(op, fold=mat is not None)
+    ab = algorithmic_bytes(op, fold=mat is not None, branch=br is not None and mat is not None)
     kern = {}
     for name, lst in ev.items():
         ms = float(np.mean([a.elapsed_time(b) for a, b in lst]))

@@ -196,6 +196,30 @@ int ssx_gather_node_rows(int dim, int nld, const double *ke, const int32_t *ns_p
                          const uint8_t *row_free, const int32_t *nodes, int64_t n_nodes, int max_len, double r,
                          const double *elastic, double *out, float *out32, void *stream);
 
+/* P2 tetrahedra (n_p 10, 11-point rule): the upper node-pair blocks of the element tangent
+ * matrices, kb (n_elem x 55 x 3 x 3): pair p = (a, b), a <= b, in row-major order over the upper
+ * triangle (p = a*10 - a(a-1)/2 + b - a), kb[e][p][i][j] = K_e[3a+i, 3b+j] = sum_q w B_a^T D B_b.
+ * The blocks below the diagonal are the transposes (the return mapping's tangent is symmetric).
+ * Optional K_r fold exactly as ssx_element_matrices_p4 (shear/lame given).  Optional branch ids per
+ * point (int8, as written by ssx_constitutive_3d for this DS; needs shear/lame): a point with id 0
+ * (elastic) has DS = D_el(shear, lame), and its contribution is formed in closed form.  Replaces
+ * the element stage of cython/assemble_tangent_values_3d.c:13-109. */
+int ssx_element_blocks_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
+                          double r, const double *shear, const double *lame, const int8_t *branch, double *kb,
+                          void *stream);
+
+/* Node gather of the node-pair blocks of ssx_element_blocks_3d into the fixed CSR pattern, with
+ * the node slots, offsets, constrained-row, blend and FP32-copy semantics of ssx_gather_node_rows
+ * (n_p 10; max_len <= 21845, one LDS copy of a node's three rows per workgroup).  nodes = the
+ * order in which the nodes are gathered (NULL: 0 .. n_nodes-1); the result does not depend on it.
+ * Together with ssx_element_blocks_3d replaces assemble_tangent_values_3d_rows_c
+ * (cython/assemble_tangent_values_3d.c:111-214) and the constrained-diagonal step of
+ * assemble_owned_tangent_values (fem/distributed_tangent.py:1559-1612). */
+int ssx_gather_node_blocks(int n_p, const double *kb, const int32_t *ns_ptr, const int32_t *ns_elem,
+                           const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
+                           const uint8_t *row_free, const int32_t *nodes, int64_t n_nodes, int max_len, double r,
+                           const double *elastic, double *out, float *out32, void *stream);
+
 /* ---- learned continuation predictor (no reference counterpart: it fills the predictor slot of
  * continuation/indirect.py:2703-2830, where the reference uses the secant of :550-566) ---- */
 

@@ -112,6 +112,10 @@ class ConstitutiveOperator:
         self._E = torch.empty((self.n_int, ns), dtype=torch.float64, device=dev)
         self._S = torch.empty((self.n_int, ns + (1 if self.dim == 2 else 0)), dtype=torch.float64, device=dev)
         self._DS = torch.empty((self.n_int, ns * ns), dtype=torch.float64, device=dev)
+        # branch ids of the return mapping that wrote _DS (P2 tetrahedra: elastic points then enter
+        # the tangent in closed form)
+        self._branch = (torch.empty(self.n_int, dtype=torch.int8, device=dev)
+                        if self.dim == 3 and pattern.n_p == 10 and pattern.ns is not None else None)
         self._fe = torch.empty(pattern.n_elem * pattern.nld, dtype=torch.float64, device=dev)
         self._F = torch.empty(pattern.n_rows, dtype=torch.float64, device=dev)
         self._ds_ready = False
@@ -206,9 +210,10 @@ class ConstitutiveOperator:
         self.pattern.strain(u, self._E)
         t1 = perf_counter()
         DS = self._DS if tangent else None
+        br = self._branch if tangent else None
         if self.dim == 3:
             _lib.call("ssx_constitutive_3d", ptr(self._E), ptr(self.c_bar), ptr(self.sin_phi), ptr(self.shear),
-                      ptr(self.lame), self.n_int, ptr(self._S), ptr(DS), None, None, stream_ptr())
+                      ptr(self.lame), self.n_int, ptr(self._S), ptr(DS), None, ptr(br), stream_ptr())
         else:
             _lib.call("ssx_constitutive_2d", ptr(self._E), ptr(self.c_bar), ptr(self.sin_phi), ptr(self.shear),
                       ptr(self.bulk), ptr(self.lame), self.n_int, ptr(self._S), ptr(DS), None, None, stream_ptr())
@@ -289,7 +294,12 @@ class ConstitutiveOperator:
         if self._owned_tangent_mat is None:
             self._owned_tangent_mat = self.pattern.new_csr(with32=False)
         t0 = perf_counter()
-        self.pattern.tangent(self._DS, self._owned_tangent_mat.values)
+        if self._branch is not None:
+            # K = K_r at r = 0, with the elastic points in closed form
+            self.pattern.tangent(self._DS, self._owned_tangent_mat.values, r=0.0, mat=(self.shear, self.lame),
+                                 branch=self._branch)
+        else:
+            self.pattern.tangent(self._DS, self._owned_tangent_mat.values)
         self.time_build_tangent_local.append(perf_counter() - t0)
         return self._owned_tangent_mat
 
@@ -310,7 +320,8 @@ class ConstitutiveOperator:
         t0 = perf_counter()
         if self.pattern.ns is not None:
             # K_r folded into the point tangents: (1-r) DS + r D_el (same operator, no K_el read)
-            self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), mat=(self.shear, self.lame))
+            self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), mat=(self.shear, self.lame),
+                                 branch=self._branch)
         else:
             self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), elastic=self.elastic_values)
         self.time_build_tangent_local.append(perf_counter() - t0)

@@ -1,0 +1,370 @@
+// Tangent assembly of P2 tetrahedra through symmetric node-pair blocks, on gfx950:
+//
+//   phase 1 (k_ke_blocks_p2)  one wavefront per element at a time; lane p < 55 owns the node pair
+//                             (a, b), a <= b, and computes the 3x3 block K_e[3a + i, 3b + j] over
+//                             the 11 points.  The point tangent D is the same for every lane of the
+//                             wavefront, so it is read with scalar loads into SGPRs and every FMA
+//                             takes its D operand from there (the gradients come from LDS); the
+//                             blocks below the diagonal are not computed.
+//   phase 2 (k_gather_blocks) one wavefront per mesh node sums the node's three rows over its
+//                             elements (element order, an LDS copy of the rows) and writes them to
+//                             the fixed CSR pattern in one coalesced pass (FP64 values and the FP32
+//                             copy).  Row 3a + i of K_e at column 3b + j is block (a, b)[i][j] when
+//                             b >= a, else block (b, a)[j][i].
+//
+// The blocks take 55 * 9 = 495 doubles per element against 900 for the full element matrix, so
+// the intermediate written and read back is 0.55x the full matrices.  The lower blocks are the
+// transposes of the upper ones: the Mohr-Coulomb tangent of the return mapping is symmetric
+// (associated flow with the reduced sin(phi)) and so is the elastic tangent.
+//
+// Block (a, b) = sum_q w_q B_a^T D_q B_b with B_a the 6x3 strain-displacement block of node a:
+// B_a[s, i] = g_a[k] for s = V[i][k] (Voigt index of the pair (i, k), engineering shears).  The
+// lane forms t[i][s'] = sum_k g_a[k] D[V[i][k]][s'] (18 values) and adds sum_l t[i][V[j][l]] g_b[l].
+// With the regularisation K_r = (1-r) K + r K_el folded in, t is scaled by w (1-r) and the elastic
+// part w r B_a^T D_el B_b = w r (lame g_a[i] g_b[j] + G (g_a[j] g_b[i] + delta_ij g_a.g_b)) enters
+// through t (D_el has the closed form lame m m^T + 2G on the normal and G on the shear diagonal).
+//
+// Summation order of a CSR entry: element blocks over the node's elements in element order (the
+// order of the reference's rows kernel), each block summed over the points in order; bitwise
+// reproducible.
+//
+// Reference behaviour: cython/assemble_tangent_values_3d.c:13-109 (element matrices), :111-214
+// (rows); fem/distributed_tangent.py:1559-1612 (constrained diagonal = 1); constitutive/problem.py:
+// 1595-1644 (K_r on the same pattern).
+#include "ssx_common.h"
+
+namespace {
+
+constexpr int kNP = 10;                       // P2 tetrahedron
+constexpr int kNQ = 11;                       // 11-point rule
+constexpr int kNPair = kNP * (kNP + 1) / 2;   // 55 node pairs a <= b
+constexpr int kBlk = 9;                       // 3x3 block
+
+// Voigt index of the index pair (i, k): [e11 e22 e33 g12 g23 g13]
+__host__ __device__ constexpr int voigt(int i, int k) {
+    return i == k ? i : ((i + k == 1) ? 3 : ((i + k == 3) ? 4 : 5));
+}
+
+// Start of row a of the upper triangle (pairs (a, a), (a, a+1), ... in row-major order).
+__host__ __device__ constexpr int pair_row_start(int a) { return a * kNP - a * (a - 1) / 2; }
+
+__device__ __forceinline__ int pair_index(int lo, int hi) { return lo * kNP - (lo * (lo - 1)) / 2 + (hi - lo); }
+
+// Lane -> node pair (a, b), a <= b: lanes 0..54 in row-major order over the upper triangle; idle
+// lanes repeat the last pair (their stores are masked).
+__device__ __forceinline__ void lane_pair(int lane, int &p, int &a, int &b) {
+    p = lane < kNPair ? lane : kNPair - 1;
+    a = 0;
+#pragma unroll
+    for (int k = 1; k < kNP; ++k) a += p >= pair_row_start(k) ? 1 : 0;
+    b = a + (p - pair_row_start(a));
+}
+
+// Point data of one element staged in LDS: gradients [c][q][node] (330), then w, shear, lame and the
+// elastic flag per point (11 each).
+constexpr int kStG = 3 * kNQ * kNP;
+constexpr int kStWords = kStG + 4 * kNQ;
+constexpr int kStRegs = (kStG + 63) / 64;   // gradient values per lane in flight
+
+// FOLD: shear/lame given, the regularisation r folded in (K_r); EL: per-point branch ids given, and a
+// point the return mapping left elastic (branch 0) has D = D_el exactly, so its block is the closed
+// form w (lame g_a g_b^T + G (g_b g_a^T + (g_a.g_b) I)) (and (1-r) D_el + r D_el = D_el).
+//
+// Persistent wavefronts walk elements e, e + grid, ...; the next element's point data is loaded
+// into registers while the current element is computed from LDS and staged into the other LDS
+// buffer afterwards; one load per lane touches every 128-byte line of the next element's D, so
+// the scalar loads of D find it in the L2.
+template <bool FOLD, bool EL>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, const double *__restrict__ w,
+               int64_t n_elem, double r, const double *__restrict__ shear, const double *__restrict__ lame,
+               const int8_t *__restrict__ branch, double *__restrict__ kb) {
+    __shared__ double sg[2][kStWords];
+    const int lane = threadIdx.x;
+    int p, a, b;
+    lane_pair(lane, p, a, b);
+    const int64_t n_int = n_elem * kNQ;
+    const double om = FOLD ? 1.0 - r : 1.0;
+    int64_t e = blockIdx.x;
+    if (e >= n_elem) return;   // whole wavefront
+
+    double rg[kStRegs], rw, rs = 0.0, rl = 0.0, touch;
+    int rb = 1;
+    // global -> registers for element ee (every index in bounds; lanes past the data read index 0
+    // or the element's last word)
+    auto fetch = [&](int64_t ee) {
+#pragma unroll
+        for (int k = 0; k < kStRegs; ++k) {
+            const int t = lane + 64 * k;
+            const int tc = t < kStG ? t : 0;
+            const int c = tc / (kNQ * kNP), rem = tc - c * (kNQ * kNP);
+            rg[k] = dphi[((int64_t)c * n_int + ee * kNQ) * kNP + rem];
+        }
+        const int lq = lane < kNQ ? lane : 0;
+        rw = w[ee * kNQ + lq];
+        if constexpr (FOLD || EL) {
+            rs = shear[ee * kNQ + lq];
+            rl = lame[ee * kNQ + lq];
+        }
+        if constexpr (EL) rb = branch[ee * kNQ + lq];
+        touch = DS[ee * (kNQ * 36) + (lane * 16 < kNQ * 36 - 1 ? lane * 16 : kNQ * 36 - 1)];
+    };
+    // registers -> LDS buffer bb
+    auto stage = [&](int bb) {
+        asm volatile("" ::"v"(touch));   // the line touch is complete (and kept) before the buffer turns
+#pragma unroll
+        for (int k = 0; k < kStRegs; ++k) {
+            const int t = lane + 64 * k;
+            if (t < kStG) sg[bb][t] = rg[k];
+        }
+        if (lane < kNQ) {
+            sg[bb][kStG + lane] = rw;
+            sg[bb][kStG + kNQ + lane] = rs;
+            sg[bb][kStG + 2 * kNQ + lane] = rl;
+            sg[bb][kStG + 3 * kNQ + lane] = rb == 0 ? 1.0 : 0.0;
+        }
+    };
+    fetch(e);
+    stage(0);
+    __syncthreads();
+    int buf = 0;
+    for (; e < n_elem; e += gridDim.x) {
+        const int64_t en = e + gridDim.x;
+        if (en < n_elem) fetch(en);
+        const double *L = sg[buf];
+        double acc[kBlk];
+#pragma unroll
+        for (int k = 0; k < kBlk; ++k) acc[k] = 0.0;
+#pragma unroll 1
+        for (int q = 0; q < kNQ; ++q) {
+            const double wq = L[kStG + q];
+            const double ha[3] = {L[0 * kNQ * kNP + q * kNP + a], L[1 * kNQ * kNP + q * kNP + a],
+                                  L[2 * kNQ * kNP + q * kNP + a]};
+            const double gb[3] = {L[0 * kNQ * kNP + q * kNP + b], L[1 * kNQ * kNP + q * kNP + b],
+                                  L[2 * kNQ * kNP + q * kNP + b]};
+            if constexpr (EL) {
+                if (L[kStG + 3 * kNQ + q] != 0.0) {   // uniform: one point for the whole wavefront
+                    const double wl = wq * L[kStG + 2 * kNQ + q], wg = wq * L[kStG + kNQ + q];
+                    const double dg = wg * fma(ha[2], gb[2], fma(ha[1], gb[1], ha[0] * gb[0]));
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const double li = wl * ha[i], gi = wg * gb[i];
+#pragma unroll
+                        for (int j = 0; j < 3; ++j) acc[3 * i + j] = fma(gi, ha[j], fma(li, gb[j], acc[3 * i + j]));
+                        acc[4 * i] += dg;
+                    }
+                    continue;
+                }
+            }
+            const double *D = DS + (e * kNQ + q) * 36;   // D[s][m] = D[s + 6 m]; uniform: scalar loads
+            const double c1 = wq * om;
+            const double ga[3] = {c1 * ha[0], c1 * ha[1], c1 * ha[2]};
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+                // t[s'] = sum_k ga[k] D[V[i][k]][s'] (row i of w (1-r) B_a^T D)
+                double t[6];
+#pragma unroll
+                for (int m = 0; m < 6; ++m) {
+                    double v = ga[0] * D[voigt(i, 0) + 6 * m];
+                    v = fma(ga[1], D[voigt(i, 1) + 6 * m], v);
+                    v = fma(ga[2], D[voigt(i, 2) + 6 * m], v);
+                    t[m] = v;
+                }
+                if constexpr (FOLD) {
+                    // + w r (B_a^T D_el) row i: lame g_a[i] on the normal strains, 2G g_a[i] more on
+                    // strain i, G g_a[k] on the shear strain of (i, k), k != i
+                    const double c2 = wq * r;
+                    const double cl = c2 * L[kStG + 2 * kNQ + q], cg = c2 * L[kStG + kNQ + q];
+                    const double li = cl * ha[i];
+#pragma unroll
+                    for (int m = 0; m < 3; ++m) t[m] += li;
+                    t[i] = fma(2.0 * cg, ha[i], t[i]);
+#pragma unroll
+                    for (int k = 0; k < 3; ++k)
+                        if (k != i) t[voigt(i, k)] = fma(cg, ha[k], t[voigt(i, k)]);
+                }
+#pragma unroll
+                for (int j = 0; j < 3; ++j) {
+                    double v = acc[3 * i + j];
+                    v = fma(t[voigt(j, 0)], gb[0], v);
+                    v = fma(t[voigt(j, 1)], gb[1], v);
+                    v = fma(t[voigt(j, 2)], gb[2], v);
+                    acc[3 * i + j] = v;
+                }
+            }
+        }
+        if (lane < kNPair) {
+            double *o = kb + (e * kNPair + p) * kBlk;
+#pragma unroll
+            for (int k = 0; k < kBlk; ++k) o[k] = acc[k];
+        }
+        if (en < n_elem) stage(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Phase 2: node gather from the blocks.  LDS holds the node's three rows (free rows share one
+// length L) plus a dummy word for constrained columns and rows.  Lane v (and v + 64) handles row
+// c = v / 30 and element column 3b + j = v % 30; for the element's local node a it reads block
+// (min(a, b), max(a, b)) at [c][j] or, below the diagonal, at [j][c].
+// ------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__ kb, const int32_t *__restrict__ ns_ptr,
+                                                      const int32_t *__restrict__ ns_elem,
+                                                      const uint8_t *__restrict__ ns_lnode,
+                                                      const int16_t *__restrict__ ns_off,
+                                                      const int32_t *__restrict__ indptr,
+                                                      const uint8_t *__restrict__ row_free,
+                                                      const int32_t *__restrict__ nodes, double r,
+                                                      const double *__restrict__ elastic, double *__restrict__ out,
+                                                      float *__restrict__ out32) {
+    constexpr int DIM = 3, NLD = 3 * kNP, NV = DIM * NLD, NPASS = (NV + 63) / 64;
+    extern __shared__ double acc[];
+    const int64_t node = nodes ? (int64_t)nodes[blockIdx.x] : (int64_t)blockIdx.x;
+    const int lane = threadIdx.x;
+    const int64_t r0 = DIM * node;
+    int fr[DIM];
+    int64_t st[DIM];
+    int L = 0;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        fr[c] = row_free[r0 + c];
+        st[c] = indptr[r0 + c];
+        const int len = indptr[r0 + c + 1] - indptr[r0 + c];
+        L = fr[c] ? len : L;
+    }
+    const int dummy = DIM * L;
+    for (int i = lane; i <= dummy; i += 64) acc[i] = 0.0;
+
+    int rowbase[NPASS], col[NPASS], bn[NPASS], cc[NPASS], jj[NPASS];
+#pragma unroll
+    for (int k = 0; k < NPASS; ++k) {
+        const int v = lane + 64 * k;
+        const bool live = v < NV;
+        const int vc = live ? v : 0;
+        const int c = vc / NLD;
+        col[k] = vc - c * NLD;
+        bn[k] = col[k] / 3;
+        jj[k] = col[k] - 3 * bn[k];
+        cc[k] = c;
+        int frc = 0;
+#pragma unroll
+        for (int t = 0; t < DIM; ++t) frc += (c == t) ? fr[t] : 0;
+        rowbase[k] = (live && frc) ? c * L : -1;
+    }
+
+    // the element slots go two at a time: both slots' loads are in flight together, then both are
+    // added in slot order (a barrier after each slot)
+    constexpr int DEPTH = 2;
+    const int32_t s0 = ns_ptr[node], s1 = ns_ptr[node + 1];
+    __syncthreads();
+    for (int32_t base = s0; base < s1; base += DEPTH) {
+        double val[DEPTH][NPASS];
+        int o[DEPTH][NPASS];
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d) {
+            const int32_t s = base + d < s1 ? base + d : s1 - 1;
+            const int64_t e = ns_elem[s];
+            const int a = ns_lnode[s];
+            const double *bp = kb + e * (kNPair * kBlk);
+            const int16_t *off = ns_off + (int64_t)s * NLD;
+#pragma unroll
+            for (int k = 0; k < NPASS; ++k) {
+                const int b = bn[k];
+                const int lo = min(a, b), hi = max(a, b);
+                const int up = b >= a ? 1 : 0;                       // 0/1 weight, no branch
+                const int ii = up * cc[k] + (1 - up) * jj[k];
+                const int jc = up * jj[k] + (1 - up) * cc[k];
+                val[d][k] = bp[pair_index(lo, hi) * kBlk + 3 * ii + jc];
+                o[d][k] = off[col[k]];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < DEPTH; ++d) {
+            if (base + d < s1) {   // uniform
+#pragma unroll
+                for (int k = 0; k < NPASS; ++k) {
+                    const int idx = (rowbase[k] >= 0 && o[d][k] >= 0) ? rowbase[k] + o[d][k] : dummy;
+                    acc[idx] += val[d][k];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const double om = 1.0 - r;
+#pragma unroll
+    for (int c = 0; c < DIM; ++c) {
+        if (fr[c]) {
+            for (int i = lane; i < L; i += 64) {
+                double v = acc[c * L + i];
+                if (elastic) v = om * v + r * elastic[st[c] + i];
+                out[st[c] + i] = v;
+                if (out32) out32[st[c] + i] = (float)v;
+            }
+        } else if (lane == 0) {
+            out[st[c]] = 1.0;  // constrained row: its diagonal only
+            if (out32) out32[st[c]] = 1.0f;
+        }
+    }
+}
+
+// Persistent grid of the block kernel: 32 wavefronts per CU (measured best of 12-32 on MI355X;
+// 4 are resident per SIMD).
+constexpr int kBlkWavesPerCU = 32;
+
+}  // namespace
+
+extern "C" {
+
+int ssx_element_blocks_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
+                          double r, const double *shear, const double *lame, const int8_t *branch, double *kb,
+                          void *stream) {
+    SSX_REQUIRE(n_p == kNP && n_q == kNQ, "ssx_element_blocks_3d: only P2 tetrahedra with the 11-point rule "
+                "(n_p 10, n_q 11), got n_p %d n_q %d", n_p, n_q);
+    SSX_REQUIRE(n_elem >= 0 && n_elem <= 0x7fffffff, "ssx_element_blocks_3d: bad element count");
+    if (n_elem == 0) return SSX_OK;
+    SSX_REQUIRE(dphi && DS && w && kb, "ssx_element_blocks_3d: null pointer");
+    SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_blocks_3d: shear and lame go together");
+    SSX_REQUIRE(branch == nullptr || shear != nullptr, "ssx_element_blocks_3d: branch ids need shear and lame");
+    hipStream_t st = ssx::as_stream(stream);
+    static int n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
+            n_cu = 256;
+    }
+    const int64_t want = (int64_t)n_cu * kBlkWavesPerCU;
+    const unsigned grid = (unsigned)(n_elem < want ? n_elem : want);
+    // FOLD (shear/lame given) applies r; with r = 0 and branch ids the fold terms vanish
+#define KB(F_, E_) \
+    hipLaunchKernelGGL((k_ke_blocks_p2<F_, E_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, branch, kb)
+    if (branch) KB(true, true);
+    else if (shear) KB(true, false);
+    else KB(false, false);
+#undef KB
+    SSX_CHECK_LAUNCH("ssx_element_blocks_3d");
+    return SSX_OK;
+}
+
+int ssx_gather_node_blocks(int n_p, const double *kb, const int32_t *ns_ptr, const int32_t *ns_elem,
+                           const uint8_t *ns_lnode, const int16_t *ns_off, const int32_t *indptr,
+                           const uint8_t *row_free, const int32_t *nodes, int64_t n_nodes, int max_len, double r,
+                           const double *elastic, double *out, float *out32, void *stream) {
+    SSX_REQUIRE(n_p == kNP, "ssx_gather_node_blocks: only P2 tetrahedra (n_p 10), got %d", n_p);
+    SSX_REQUIRE(n_nodes >= 0 && n_nodes <= 0x7fffffff && max_len >= 0 && max_len < 32768,
+                "ssx_gather_node_blocks: bad sizes");
+    if (n_nodes == 0) return SSX_OK;
+    SSX_REQUIRE(kb && ns_ptr && ns_elem && ns_lnode && ns_off && indptr && row_free && out,
+                "ssx_gather_node_blocks: null pointer");
+    const size_t lds = ((size_t)3 * max_len + 1) * sizeof(double);
+    SSX_REQUIRE(lds <= 65536, "ssx_gather_node_blocks: rows of %d entries exceed the LDS", max_len);
+    hipStream_t st = ssx::as_stream(stream);
+    hipLaunchKernelGGL(k_gather_blocks, dim3((unsigned)n_nodes), dim3(64), lds, st, kb, ns_ptr, ns_elem, ns_lnode,
+                       ns_off, indptr, row_free, nodes, r, elastic, out, out32);
+    SSX_CHECK_LAUNCH("ssx_gather_node_blocks");
+    return SSX_OK;
+}
+
+}  // extern "C"

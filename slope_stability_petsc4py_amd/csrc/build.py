@@ -27,6 +27,7 @@ SOURCES = [
     "p4_mfma.hip",
     "mlp.hip",
     "node_assembly.hip",
+    "block_assembly.hip",
     "probe.hip",
 ]
 

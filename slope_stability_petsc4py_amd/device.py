@@ -140,6 +140,7 @@ class DevicePattern:
             groups = ns.buckets() if ns.max_len > 640 else []
             self.ns_groups = [(to_dev(ids), ml) for ids, ml in groups] if len(groups) > 1 else []
         self._ke = None
+        self._kb = None
         self.force_gather = "nodes"     # "nodes" (node slots, 3D) or "rows" (row slots)
 
     @property
@@ -192,6 +193,23 @@ class DevicePattern:
                   self.n_q, float(r), ptr(sh), ptr(la), ptr(ke), stream_ptr())
         return ke
 
+    def element_blocks_3d(self, DS: torch.Tensor, r: float = 0.0, mat=None,
+                          branch: torch.Tensor | None = None) -> torch.Tensor:
+        """(n_elem, 55, 3, 3) upper node-pair blocks of the P2 element tangents; `branch` = the return
+        mapping's per-point branch ids for this DS (elastic points then take the closed form)."""
+        if self._kb is None:
+            self._kb = torch.empty((self.n_elem, 55, 3, 3), dtype=torch.float64, device=DS.device)
+        sh, la = (None, None) if mat is None else mat
+        _lib.call("ssx_element_blocks_3d", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
+                  self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
+        return self._kb
+
+    def gather_blocks(self, kb: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
+                      elastic: torch.Tensor | None = None) -> None:
+        _lib.call("ssx_gather_node_blocks", self.n_p, ptr(kb), ptr(self.ns_ptr), ptr(self.ns_elem),
+                  ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), None,
+                  self.host.n_nodes, self.ns.max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
+
     def gather_nodes(self, ke: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
                      elastic: torch.Tensor | None = None) -> None:
         groups = self.ns_groups or [(None, self.ns.max_len)]
@@ -201,17 +219,25 @@ class DevicePattern:
                       ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), ptr(ids), n,
                       max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
 
-    # 3D tetrahedra: "nodes" (element matrices + node gather, the default) or "rows" (fused row
-    # kernel, the 2D path)
+    # 3D tetrahedra: "nodes" (element matrices + node gather, the default; P2 meshes take the
+    # symmetric node-pair blocks instead of full element matrices) or "rows" (fused row kernel,
+    # the 2D path)
     tangent_method = "nodes"
 
     def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
-                elastic: torch.Tensor | None = None, mat=None) -> None:
+                elastic: torch.Tensor | None = None, mat=None, branch: torch.Tensor | None = None) -> None:
         """Tangent values on the pattern.  K_r = (1-r) K + r K_el either from the assembled elastic
-        values (`elastic`) or folded into the point tangents (`mat` = (shear, lame) per point)."""
+        values (`elastic`) or folded into the point tangents (`mat` = (shear, lame) per point).
+        `branch` (P2 tetrahedra, with `mat`): the return mapping's branch ids for this DS."""
         if mat is not None and self.ns is None:
             raise ValueError("the K_r fold needs the node-gather path (3D tetrahedra)")
+        if branch is not None and mat is None:
+            raise ValueError("branch ids need the material (mat = (shear, lame))")
         if self.ns is not None and (self.n_p == 35 or self.tangent_method == "nodes"):
+            if self.n_p == 10 and self.n_q == 11:
+                kb = self.element_blocks_3d(DS, r, mat, branch)
+                self.gather_blocks(kb, out, out32, r, None if mat is not None else elastic)
+                return
             if self.n_p == 35:
                 ke = self.element_matrices_p4(DS, r, mat)
             else:

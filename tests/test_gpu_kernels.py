@@ -462,6 +462,91 @@ def test_p2_element_kernels_vs_oracle(gpu, rule):
     assert _rel(got, exp) < 1e-12
 
 
+def _upper_blocks(ke):
+    """(n_e, 55, 3, 3) upper node-pair blocks (a <= b, row-major) of (n_e, 30, 30) element matrices."""
+    kef = ke.reshape(ke.shape[0], 10, 3, 10, 3)
+    a, b = np.triu_indices(10)
+    return np.ascontiguousarray(kef[:, a, :, b, :].transpose(1, 0, 2, 3))
+
+
+@pytest.mark.parametrize("fold", [False, True])
+def test_p2_element_blocks_vs_oracle(gpu, fold):
+    """Upper node-pair blocks of the P2 element matrices (persistent kernel, odd element count so
+    the last sweep of the grid is partial) == the upper blocks of the oracle's element matrices,
+    with and without the K_r fold; repeatable bit for bit."""
+    print(f"case: P2 element blocks, fold={fold}", flush=True)
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=7.5, nz=3)
+    geom = fem.geometry(m.coord, m.elem, "P2")
+    n_e = geom.n_elem - 1 if geom.n_elem % 2 == 0 else geom.n_elem
+    dev = gpu
+    rng = np.random.default_rng(47)
+    A = rng.normal(size=(geom.n_int, 6, 6))
+    DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
+    G = rng.uniform(1e3, 2e4, geom.n_int)
+    lam = rng.uniform(1e3, 3e4, geom.n_int)
+    vol = np.array([1, 1, 1, 0, 0, 0.0])
+    idn = np.diag([1, 1, 1, 0.5, 0.5, 0.5])
+    Del = (lam[:, None, None] * np.outer(vol, vol)[None] + 2 * G[:, None, None] * idn[None]).reshape(-1, 36)
+    r = 0.25 if fold else 0.0
+    n_int = n_e * geom.n_q
+    dphi = np.ascontiguousarray(geom.dphi[:, :n_int, :])
+    wt, DS, G, lam, Del = geom.weight[:n_int], DS[:n_int], G[:n_int], lam[:n_int], Del[:n_int]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    args = [t(dphi), t(DS), t(wt), t(G), t(lam)]
+    kb = [torch.full((n_e, 55, 3, 3), np.nan, dtype=torch.float64, device=dev) for _ in range(2)]
+    for out in kb:
+        _lib.call("ssx_element_blocks_3d", 10, int(args[0].data_ptr()), int(args[1].data_ptr()),
+                  int(args[2].data_ptr()), n_e, geom.n_q, r, int(args[3].data_ptr()) if fold else None,
+                  int(args[4].data_ptr()) if fold else None, None, int(out.data_ptr()), 0)
+    torch.cuda.synchronize()
+    exp = _upper_blocks(oracle.element_matrices_3d(dphi, (1 - r) * DS + r * Del, wt, n_e, geom.n_q))
+    got = kb[0].cpu().numpy()
+    assert np.isfinite(got).all()
+    assert _rel(got, exp) < 1e-12
+    assert torch.equal(kb[0], kb[1])
+
+
+def test_p2_tangent_with_branch_ids_vs_oracle(gpu):
+    """Tangent from the return mapping's own DS and branch ids (elastic points in closed form) on a
+    stratified strain field covering every branch: K_r and K equal the oracle rows kernel on the
+    same DS to 1e-12, agree with the path without branch ids, and repeat bit for bit."""
+    print("case: P2 tangent with branch ids", flush=True)
+    m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=3)
+    geom = fem.geometry(m.coord, m.elem, "P2")
+    pat = pattern.build_owned_pattern(m.elem, m.q_mask)
+    dp = DevicePattern(pat, geom)
+    dev = dp.indptr.device
+    n = geom.n_int
+    E, _b = W.stratified_strains(n, seed=5)
+    cb, sp = np.full(n, 4 * np.sqrt(3)), np.full(n, 1 / np.sqrt(3))
+    G, lam = np.full(n, W.HOMO.shear), np.full(n, W.HOMO.lame)
+    S, DSd, br = constitutive_problem_3d(E, cb, sp, G, lam, return_tangent=True, return_branch=True)
+    hist = np.bincount(br.cpu().numpy().astype(np.int64), minlength=5)
+    assert (hist > 0).all(), hist
+    DS = DSd.cpu().numpy()
+    Gd, lamd = (torch.from_numpy(a).to(dev) for a in (G, lam))
+    vol = np.array([1, 1, 1, 0, 0, 0.0])
+    idn = np.diag([1, 1, 1, 0.5, 0.5, 0.5])
+    Del = (lam[:, None, None] * np.outer(vol, vol)[None] + 2 * G[:, None, None] * idn[None]).reshape(-1, 36)
+    Kt = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+    Kel = oracle.tangent_3d_rows(geom.dphi, Del, geom.weight, pat, geom.n_q)
+    for r in (0.0, 1e-4, 0.5):
+        exp = (1 - r) * Kt + r * Kel
+        exp[pat.constrained_diag_positions] = 1.0
+        K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+        K32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev)
+        dp.tangent(DSd, K, K32, r=r, mat=(Gd, lamd), branch=br)
+        assert _rel(K.cpu().numpy(), exp) < 1e-12, r
+        np.testing.assert_array_equal(K32.cpu().numpy(), K.cpu().numpy().astype(np.float32))
+        K2 = torch.empty_like(K)
+        dp.tangent(DSd, K2, r=r, mat=(Gd, lamd), branch=br)
+        assert torch.equal(K, K2)
+        dp.tangent(DSd, K2, r=r, mat=(Gd, lamd))
+        assert _rel(K.cpu().numpy(), K2.cpu().numpy()) < 1e-13
+    with pytest.raises(ValueError):
+        dp.tangent(DSd, K2, r=0.1, branch=br)
+
+
 @pytest.mark.parametrize("n", [1, 64, 200, 1000])
 @pytest.mark.parametrize("f64", [False, True])
 def test_sym_tiles_gemv_vs_numpy(gpu, n, f64):
