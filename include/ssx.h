@@ -175,8 +175,9 @@ int ssx_element_matrices_p4(const double *dphi, const double *DS, const double *
                             const double *shear, const double *lame, double *ke, void *stream);
 
 /* P1/P2 tetrahedra (n_p 4 or 10): element tangent matrices ke (n_elem x 3n_p x 3n_p, row-major):
- * persistent wavefronts with the next element's point data in flight (11-point P2 and 1-point P1
- * rules), one wavefront per element otherwise; optional K_r fold exactly as ssx_element_matrices_p4.  Replaces the
+ * persistent wavefronts with the next element's point data in flight for the 1-point P1 rule, one
+ * wavefront per element otherwise (P2 elements at any rule, e.g. the P2 level of the P4 multigrid
+ * at the 24 P4 points); optional K_r fold exactly as ssx_element_matrices_p4.  Replaces the
  * element stage of cython/assemble_tangent_values_3d.c:13-109. */
 int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const double *w, int64_t n_elem, int n_q,
                             double r, const double *shear, const double *lame, double *ke, void *stream);

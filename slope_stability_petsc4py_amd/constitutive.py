@@ -132,7 +132,7 @@ class ConstitutiveOperator:
     def from_mesh(cls, mesh, materials, *, Davis_type: str = "B", fp32_shadow: bool = True):
         geom = geometry(mesh.coord, mesh.elem, mesh.elem_type)
         pat = build_owned_pattern(mesh.elem, mesh.q_mask)
-        dpat = DevicePattern(pat, geom)
+        dpat = DevicePattern(pat, geom, coord=mesh.coord)
         c0, phi, psi, shear, bulk, lame, gamma = expand_materials(mesh.material, geom.n_q, materials)
         op = cls(dpat, c0, phi, psi, shear, bulk, lame, Davis_type=Davis_type, q_mask=mesh.q_mask,
                  fp32_shadow=fp32_shadow)

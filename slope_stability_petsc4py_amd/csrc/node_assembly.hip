@@ -115,7 +115,8 @@ __global__ void __launch_bounds__(64) k_ke_3d(const double *__restrict__ dphi, c
 }
 
 // ------------------------------------------------------------------------------------------
-// Phase 1, pipelined form for P2 (NP = 10, NQ = 11) and P1 (NP = 4, NQ = 1): persistent
+// Phase 1, pipelined form for P1 (NP = 4, NQ = 1; P2 meshes with the 11-point rule take the
+// node-pair blocks of block_assembly.hip instead): persistent
 // wavefronts walk elements e, e + grid, ...; while one element is computed from LDS, the next
 // element's point data (gradients, D, w and the material) is already in flight into registers
 // and is written to the other LDS buffer afterwards, so the global-memory latency is paid once
@@ -446,7 +447,7 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
     SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_matrices_3d: shear and lame go together");
     SSX_REQUIRE(n_elem <= 0x7fffffff, "ssx_element_matrices_3d: too many elements for one launch");
     hipStream_t st = ssx::as_stream(stream);
-    if ((n_p == 10 && n_q == 11) || (n_p == 4 && n_q == 1)) {
+    if (n_p == 4 && n_q == 1) {
         static int n_cu = 0;
         if (n_cu == 0) {
             int dev = 0;
@@ -459,11 +460,7 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
         const bool fold = shear != nullptr;
 #define PIPE(NP_, NQ_, F_) \
     hipLaunchKernelGGL((k_ke_3d_pipe<NP_, NQ_, F_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, ke)
-        if (n_p == 10) {
-            if (fold) PIPE(10, 11, true); else PIPE(10, 11, false);
-        } else {
-            if (fold) PIPE(4, 1, true); else PIPE(4, 1, false);
-        }
+        if (fold) PIPE(4, 1, true); else PIPE(4, 1, false);
 #undef PIPE
         SSX_CHECK_LAUNCH("ssx_element_matrices_3d");
         return SSX_OK;

@@ -101,7 +101,7 @@ class DeviceCSR:
 class DevicePattern:
     """The fixed pattern, row slots and per-point geometry resident in HBM."""
 
-    def __init__(self, pat: OwnedPattern, geom: Geometry):
+    def __init__(self, pat: OwnedPattern, geom: Geometry, coord: np.ndarray | None = None):
         self.host = pat
         self.geom = geom
         self.dim = pat.dim
@@ -139,6 +139,11 @@ class DevicePattern:
             # reach 2,457 entries against 105 for interior nodes); shorter meshes keep one launch
             groups = ns.buckets() if ns.max_len > 640 else []
             self.ns_groups = [(to_dev(ids), ml) for ids, ml in groups] if len(groups) > 1 else []
+            # launch order of the block gather (P2): Morton order of the nodes over the XCDs
+            self.gather_ids = None
+            if coord is not None and self.n_p == 10 and self.n_q == 11:
+                from .pattern import gather_order
+                self.gather_ids = to_dev(gather_order(coord))
         self._ke = None
         self._kb = None
         self.force_gather = "nodes"     # "nodes" (node slots, 3D) or "rows" (row slots)
@@ -207,7 +212,7 @@ class DevicePattern:
     def gather_blocks(self, kb: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
                       elastic: torch.Tensor | None = None) -> None:
         _lib.call("ssx_gather_node_blocks", self.n_p, ptr(kb), ptr(self.ns_ptr), ptr(self.ns_elem),
-                  ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), None,
+                  ptr(self.ns_lnode), ptr(self.ns_off), ptr(self.indptr), ptr(self.row_free), ptr(self.gather_ids),
                   self.host.n_nodes, self.ns.max_len, float(r), ptr(elastic), ptr(out), ptr(out32), stream_ptr())
 
     def gather_nodes(self, ke: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,

@@ -426,10 +426,9 @@ def test_node_gather_kr_fold_vs_oracle(gpu, elem_type):
 
 @pytest.mark.parametrize("rule", ["P2", "P4"])
 def test_p2_element_kernels_vs_oracle(gpu, rule):
-    """P2 element matrices with K_r folded, on an odd element count: the pipelined persistent
-    kernel (11-point rule; the last sweep of the grid is partial) and the one-wavefront-per-element
-    kernel (P2 elements at the 24 points of the P4 rule, as the P4 multigrid's P2 level) both give
-    the oracle's regularised element matrices."""
+    """Full P2 element matrices with K_r folded, on an odd element count (one wavefront per element;
+    the 11-point rule, and the 24 points of the P4 rule as the P4 multigrid's P2 level) give the
+    oracle's regularised element matrices."""
     print(f"case: P2 element kernel, {rule} points", flush=True)
     from slope_stability_petsc4py_amd import _lib
     from slope_stability_petsc4py_amd.fem import quadrature_3d

@@ -143,3 +143,24 @@ def test_oracle_matches_compiled_reference_assembly(small3d):
 def test_int32_overflow_guard():
     with pytest.raises(OverflowError):
         pattern._check_int32("nnz", 2**31)
+
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 1003])
+def test_gather_order_is_a_permutation_dealt_over_xcds(n):
+    """The block gather's launch order visits every node once; workgroup k of the order sits in
+    run k mod 8 of the Morton-sorted nodes, and each run is contiguous along the curve."""
+    X = np.random.default_rng(n).random((3, n))
+    order = pattern.gather_order(X)
+    assert np.array_equal(np.sort(order), np.arange(n))
+    # runs per XCD: consecutive workgroups of one XCD take consecutive Morton positions
+    q = np.minimum((X - X.min(axis=1, keepdims=True)) / np.maximum(np.ptp(X, axis=1, keepdims=True), 1e-300) * 1024,
+                   1023).astype(np.int64)
+    code = np.zeros(n, dtype=np.int64)
+    for bit in range(10):
+        for d in range(3):
+            code |= ((q[d] >> bit) & 1) << (3 * bit + d)
+    rank = np.empty(n, dtype=np.int64)
+    rank[np.argsort(code, kind="stable")] = np.arange(n)
+    for x in range(8):
+        r = rank[order[x::8]]
+        assert np.all(np.diff(r) == 1)
