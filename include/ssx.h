@@ -184,7 +184,8 @@ int ssx_element_matrices_3d(int n_p, const double *dphi, const double *DS, const
 
 /* Node gather of element matrices into the fixed CSR pattern: one wavefront per mesh node sums the
  * node's dim rows over its elements (node slots ns_ptr/ns_elem/ns_lnode, element order) at the
- * int16 column offsets ns_off (n_slots x nld, relative to the row start, -1 = constrained column),
+ * int16 column offsets ns_off (n_slots x nld, relative to the row start; a constrained column
+ * holds the row length L),
  * sets constrained rows to their unit diagonal, optionally blends (1-r) K + r elastic, and writes
  * FP64 values and an optional FP32 copy.  nodes = the node ids to gather (n_nodes of them; NULL:
  * nodes 0 .. n_nodes-1), max_len = longest free row among them (< 32768; it sizes the LDS of a
@@ -211,7 +212,7 @@ int ssx_element_blocks_3d(int n_p, const double *dphi, const double *DS, const d
 
 /* Node gather of the node-pair blocks of ssx_element_blocks_3d into the fixed CSR pattern, with
  * the node slots, offsets, constrained-row, blend and FP32-copy semantics of ssx_gather_node_rows
- * (n_p 10; max_len <= 21845, one LDS copy of a node's three rows per workgroup).  nodes = the
+ * (n_p 10; max_len <= 2729, one LDS copy of a node's three rows per workgroup).  nodes = the
  * order in which the nodes are gathered (NULL: 0 .. n_nodes-1); the result does not depend on it.
  * Together with ssx_element_blocks_3d replaces assemble_tangent_values_3d_rows_c
  * (cython/assemble_tangent_values_3d.c:111-214) and the constrained-diagonal step of

@@ -48,8 +48,6 @@ __host__ __device__ constexpr int voigt(int i, int k) {
 // Start of row a of the upper triangle (pairs (a, a), (a, a+1), ... in row-major order).
 __host__ __device__ constexpr int pair_row_start(int a) { return a * kNP - a * (a - 1) / 2; }
 
-__device__ __forceinline__ int pair_index(int lo, int hi) { return lo * kNP - (lo * (lo - 1)) / 2 + (hi - lo); }
-
 // Lane -> node pair (a, b), a <= b: lanes 0..54 in row-major order over the upper triangle; idle
 // lanes repeat the last pair (their stores are masked).
 __device__ __forceinline__ void lane_pair(int lane, int &p, int &a, int &b) {
@@ -205,10 +203,22 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
 }
 
 // ------------------------------------------------------------------------------------------
-// Phase 2: node gather from the blocks.  LDS holds the node's three rows (free rows share one
-// length L) plus a dummy word for constrained columns and rows.  Lane v (and v + 64) handles row
-// c = v / 30 and element column 3b + j = v % 30; for the element's local node a it reads block
-// (min(a, b), max(a, b)) at [c][j] or, below the diagonal, at [j][c].
+// Phase 2: node gather from the blocks.  LDS holds the node's three rows, each of length L + 1
+// (free rows share one length L; word L of a row absorbs its constrained columns, whose offset is
+// L), plus one spare word.  Lane v (and v + 64) handles row c = v / 30 and element column 3b + j =
+// v % 30.  For the element's local node a, K_e[3a + c, 3b + j] is block (a, b)[c][j] when b >= a,
+// else block (b, a)[j][c]; its offset in the element's 495 words splits into a part that depends
+// only on a (the same for the wavefront) and a part that depends only on the lane:
+//   b >= a:  9 (p0(a) - a) + (9b + 3c + j),      b < a:  9a + (9 (p0(b) - b) + 3j + c),
+// with p0(x) the first pair of row x of the upper triangle.
+//
+// The node's slot metadata (element, local node) for up to 64 slots comes in one vector load and
+// each slot's pair is read from its lane (readlane).  Slots go two at a time: both slots' loads are
+// issued, then both are added in slot order.  Every add is unconditional -- a slot past the node's
+// last one and the lanes past the 90 values add into the spare word -- so the loads feed no branch
+// and stay in flight together (a conditional add lets the compiler sink each slot's loads into its
+// branch, which serialises them).  LDS operations of one wavefront complete in order, so slot
+// order needs no barrier beyond the wavefront's own.
 // ------------------------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__ kb, const int32_t *__restrict__ ns_ptr,
                                                       const int32_t *__restrict__ ns_elem,
@@ -219,7 +229,7 @@ __global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__
                                                       const int32_t *__restrict__ nodes, double r,
                                                       const double *__restrict__ elastic, double *__restrict__ out,
                                                       float *__restrict__ out32) {
-    constexpr int DIM = 3, NLD = 3 * kNP, NV = DIM * NLD, NPASS = (NV + 63) / 64;
+    constexpr int DIM = 3, NLD = 3 * kNP, NV = DIM * NLD, NPASS = (NV + 63) / 64, DEPTH = 2;
     extern __shared__ double acc[];
     const int64_t node = nodes ? (int64_t)nodes[blockIdx.x] : (int64_t)blockIdx.x;
     const int lane = threadIdx.x;
@@ -234,62 +244,57 @@ __global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__
         const int len = indptr[r0 + c + 1] - indptr[r0 + c];
         L = fr[c] ? len : L;
     }
-    const int dummy = DIM * L;
-    for (int i = lane; i <= dummy; i += 64) acc[i] = 0.0;
+    const int32_t s0 = ns_ptr[node], s1 = ns_ptr[node + 1];
+    const int L1 = L + 1;
+    const int spare = DIM * L1;
+    for (int i = lane; i <= spare; i += 64) acc[i] = 0.0;
 
-    int rowbase[NPASS], col[NPASS], bn[NPASS], cc[NPASS], jj[NPASS];
+    int rb[NPASS], col[NPASS], bn[NPASS], pup[NPASS], plo[NPASS];
 #pragma unroll
     for (int k = 0; k < NPASS; ++k) {
         const int v = lane + 64 * k;
-        const bool live = v < NV;
-        const int vc = live ? v : 0;
+        const int vc = v < NV ? v : NV - 1;   // lanes past the values read the last one (added to the spare word)
         const int c = vc / NLD;
         col[k] = vc - c * NLD;
-        bn[k] = col[k] / 3;
-        jj[k] = col[k] - 3 * bn[k];
-        cc[k] = c;
-        int frc = 0;
-#pragma unroll
-        for (int t = 0; t < DIM; ++t) frc += (c == t) ? fr[t] : 0;
-        rowbase[k] = (live && frc) ? c * L : -1;
+        const int b = col[k] / 3, j = col[k] - 3 * b;
+        bn[k] = b;
+        pup[k] = kBlk * b + 3 * c + j;
+        plo[k] = kBlk * (pair_row_start(b) - b) + 3 * j + c;
+        rb[k] = v < NV ? c * L1 : -1;
     }
-
-    // the element slots go two at a time: both slots' loads are in flight together, then both are
-    // added in slot order (a barrier after each slot)
-    constexpr int DEPTH = 2;
-    const int32_t s0 = ns_ptr[node], s1 = ns_ptr[node + 1];
     __syncthreads();
-    for (int32_t base = s0; base < s1; base += DEPTH) {
-        double val[DEPTH][NPASS];
-        int o[DEPTH][NPASS];
+    for (int32_t cs = s0; cs < s1; cs += 64) {
+        const int nc = (s1 - cs) < 64 ? (s1 - cs) : 64;
+        const int ml = lane < nc ? lane : nc - 1;
+        const int my_e = ns_elem[cs + ml];
+        const int my_a = ns_lnode[cs + ml];
+        for (int base = 0; base < nc; base += DEPTH) {
+            double val[DEPTH][NPASS];
+            int o[DEPTH][NPASS];
 #pragma unroll
-        for (int d = 0; d < DEPTH; ++d) {
-            const int32_t s = base + d < s1 ? base + d : s1 - 1;
-            const int64_t e = ns_elem[s];
-            const int a = ns_lnode[s];
-            const double *bp = kb + e * (kNPair * kBlk);
-            const int16_t *off = ns_off + (int64_t)s * NLD;
+            for (int d = 0; d < DEPTH; ++d) {
+                const int k = base + d < nc ? base + d : nc - 1;
+                const int64_t e = __builtin_amdgcn_readlane(my_e, k);
+                const int a = __builtin_amdgcn_readlane(my_a, k);
+                const double *bp = kb + e * (kNPair * kBlk);
+                const int up0 = kBlk * (pair_row_start(a) - a), lo0 = kBlk * a;
+                const int16_t *off = ns_off + (int64_t)(cs + k) * NLD;
 #pragma unroll
-            for (int k = 0; k < NPASS; ++k) {
-                const int b = bn[k];
-                const int lo = min(a, b), hi = max(a, b);
-                const int up = b >= a ? 1 : 0;                       // 0/1 weight, no branch
-                const int ii = up * cc[k] + (1 - up) * jj[k];
-                const int jc = up * jj[k] + (1 - up) * cc[k];
-                val[d][k] = bp[pair_index(lo, hi) * kBlk + 3 * ii + jc];
-                o[d][k] = off[col[k]];
-            }
-        }
-#pragma unroll
-        for (int d = 0; d < DEPTH; ++d) {
-            if (base + d < s1) {   // uniform
-#pragma unroll
-                for (int k = 0; k < NPASS; ++k) {
-                    const int idx = (rowbase[k] >= 0 && o[d][k] >= 0) ? rowbase[k] + o[d][k] : dummy;
-                    acc[idx] += val[d][k];
+                for (int kk = 0; kk < NPASS; ++kk) {
+                    val[d][kk] = bp[bn[kk] >= a ? up0 + pup[kk] : lo0 + plo[kk]];
+                    o[d][kk] = off[col[kk]];
                 }
             }
-            __syncthreads();
+#pragma unroll
+            for (int d = 0; d < DEPTH; ++d) {
+                const bool slot_ok = base + d < nc;   // uniform
+#pragma unroll
+                for (int kk = 0; kk < NPASS; ++kk) {
+                    const int idx = (slot_ok && rb[kk] >= 0) ? rb[kk] + o[d][kk] : spare;
+                    acc[idx] += val[d][kk];
+                }
+                __syncthreads();
+            }
         }
     }
     const double om = 1.0 - r;
@@ -297,7 +302,7 @@ __global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__
     for (int c = 0; c < DIM; ++c) {
         if (fr[c]) {
             for (int i = lane; i < L; i += 64) {
-                double v = acc[c * L + i];
+                double v = acc[c * L1 + i];
                 if (elastic) v = om * v + r * elastic[st[c] + i];
                 out[st[c] + i] = v;
                 if (out32) out32[st[c] + i] = (float)v;
@@ -358,7 +363,7 @@ int ssx_gather_node_blocks(int n_p, const double *kb, const int32_t *ns_ptr, con
     if (n_nodes == 0) return SSX_OK;
     SSX_REQUIRE(kb && ns_ptr && ns_elem && ns_lnode && ns_off && indptr && row_free && out,
                 "ssx_gather_node_blocks: null pointer");
-    const size_t lds = ((size_t)3 * max_len + 1) * sizeof(double);
+    const size_t lds = ((size_t)3 * (max_len + 1) + 1) * sizeof(double);
     SSX_REQUIRE(lds <= 65536, "ssx_gather_node_blocks: rows of %d entries exceed the LDS", max_len);
     hipStream_t st = ssx::as_stream(stream);
     hipLaunchKernelGGL(k_gather_blocks, dim3((unsigned)n_nodes), dim3(64), lds, st, kb, ns_ptr, ns_elem, ns_lnode,

@@ -300,8 +300,9 @@ __global__ void __launch_bounds__(64) k_ke_3d_pipe(const double *__restrict__ dp
 }
 
 // ------------------------------------------------------------------------------------------
-// Phase 2: node gather.  One wavefront per node; LDS holds the node's DIM rows (free rows share
-// one length L) plus a dummy word that absorbs the updates of constrained columns and rows.  Each
+// Phase 2: node gather.  One wavefront per node; LDS holds the node's DIM rows, each of length
+// L + 1 (free rows share one length L; word L of a row absorbs its constrained columns, whose
+// offset is L), plus a spare word that absorbs the updates of constrained rows.  Each
 // element contributes its DIM contiguous rows DIM*li .. DIM*li + DIM-1 (DIM*NLD values, read
 // coalesced); lane p handles value p (and p + 64, ...), i.e. row c = p / NLD, column b = p % NLD.
 // Within one element the targets are distinct (distinct columns); elements are applied one after
@@ -334,10 +335,11 @@ __global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ 
         const int len = indptr[r0 + c + 1] - indptr[r0 + c];
         L = fr[c] ? len : L;
     }
-    const int dummy = DIM * L;
-    for (int i = lane; i <= dummy; i += 64) acc[i] = 0.0;
+    const int L1 = L + 1;
+    const int spare = DIM * L1;
+    for (int i = lane; i <= spare; i += 64) acc[i] = 0.0;
 
-    // per-pass constant part of the LDS target: c*L for a free row, the dummy word otherwise
+    // per-pass constant part of the LDS target: c*(L+1) for a free row, -1 (the spare word) otherwise
     int rowbase[NPASS], col[NPASS];
     bool live[NPASS];
 #pragma unroll
@@ -350,7 +352,7 @@ __global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ 
         int frc = 0;
 #pragma unroll
         for (int cc = 0; cc < DIM; ++cc) frc += (c == cc) ? fr[cc] : 0;
-        rowbase[k] = (live[k] && frc) ? c * L : -1;
+        rowbase[k] = (live[k] && frc) ? c * L1 : -1;
     }
 
     const int32_t s0 = ns_ptr[node], s1 = ns_ptr[node + 1];
@@ -376,7 +378,7 @@ __global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ 
         load(s + 1 < s1 ? s + 1 : s, vn, on);
 #pragma unroll
         for (int k = 0; k < NPASS; ++k) {
-            const int idx = (rowbase[k] >= 0 && o[k] >= 0) ? rowbase[k] + o[k] : dummy;
+            const int idx = rowbase[k] >= 0 ? rowbase[k] + o[k] : spare;
             acc[idx] += v[k];
         }
         __syncthreads();
@@ -391,7 +393,7 @@ __global__ void __launch_bounds__(64) k_gather_nodes(const double *__restrict__ 
     for (int c = 0; c < DIM; ++c) {
         if (fr[c]) {
             for (int i = lane; i < L; i += 64) {
-                double val = acc[c * L + i];
+                double val = acc[c * L1 + i];
                 if (elastic) val = om * val + r * elastic[st[c] + i];
                 out[st[c] + i] = val;
                 if (out32) out32[st[c] + i] = (float)val;
@@ -408,7 +410,7 @@ int launch_gather(const double *ke, const int32_t *ns_ptr, const int32_t *ns_ele
                   const int16_t *ns_off, const int32_t *indptr, const uint8_t *row_free, const int32_t *nodes,
                   int64_t n_nodes, int max_len, double r, const double *elastic, double *out, float *out32,
                   hipStream_t st) {
-    const size_t lds = ((size_t)DIM * max_len + 1) * sizeof(double);
+    const size_t lds = ((size_t)DIM * (max_len + 1) + 1) * sizeof(double);
     if (lds > 65536) {
         static int max_lds = -1;
         if (max_lds < 0) {

@@ -189,9 +189,11 @@ class NodeSlots:
 
     For every node with at least one free component: the elements touching it in element order
     (the row-slot order of its free rows), the node's local index in each, and the CSR position of
-    every local column relative to the start of the node's rows (int16, -1 for constrained
-    columns).  All free rows of a node hold the same column list, so one offset table serves them
-    all.  Nodes without free components get no slots (their rows are unit rows)."""
+    every local column relative to the start of the node's rows (int16; a constrained column holds
+    the row length L, which is the spare word after each row in the gathers' LDS copy of the rows,
+    so the gathers index it without a test).  All free rows of a node hold the same column list, so
+    one offset table serves them all.  Nodes without free components get no slots (their rows are
+    unit rows)."""
 
     ptr: np.ndarray        # (n_nodes+1,) int32
     elem: np.ndarray       # (n_slots,) int32
@@ -255,13 +257,13 @@ def node_slots(pat: OwnedPattern) -> NodeSlots:
     sel = np.repeat(rsp[r0] - ptr[:-1], cnt) + np.arange(n)                    # row-slot index per node slot
     base = np.repeat(ip[r0], cnt)
     pos = pat.slot_positions(sel)
-    off = np.where(pos >= 0, pos - base[:, None], -1)
     row_len = np.diff(ip)
+    node_len = np.repeat(row_len[r0], cnt)
+    off = np.where(pos >= 0, pos - base[:, None], node_len[:, None])
     max_len = int(row_len[pat.row_free.astype(bool)].max()) if pat.row_free.any() else 1
     if max_len >= 32768:
         raise OverflowError(f"row length {max_len} exceeds the int16 column offsets")
-    node_len = np.repeat(row_len[r0], cnt)
-    if n and (off.max(axis=1) >= node_len).any():
+    if n and ((off.max(axis=1) > node_len).any() or (off.min(axis=1) < 0).any()):
         raise AssertionError("node-slot column offset outside its row")
     return NodeSlots(ptr=ptr.astype(np.int32), elem=pat.slot_elem[sel].astype(np.int32),
                      lnode=(pat.slot_lrow[sel] // dim).astype(np.uint8),

@@ -136,7 +136,8 @@ def test_xyz_order_is_x_major():
 
 def test_node_slots_share_offsets_across_free_rows():
     """One int16 offset table per (node, element) serves every free row of the node: it equals
-    slot_pos - row start for each of them; slots are in element order."""
+    slot_pos - row start for each of them (the row length for a constrained column); slots are in
+    element order."""
     from slope_stability_petsc4py_amd import pattern
     m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=3)
     pat = pattern.build_owned_pattern(m.elem, m.q_mask)
@@ -157,7 +158,7 @@ def test_node_slots_share_offsets_across_free_rows():
             np.testing.assert_array_equal(pat.slot_elem[s0:s1], ns.elem[a:b])
             np.testing.assert_array_equal(pat.slot_lrow[s0:s1] // dim, ns.lnode[a:b])
             pos = pat.slot_pos[s0:s1].astype(np.int64)
-            exp = np.where(pos >= 0, pos - ip[row], -1)
+            exp = np.where(pos >= 0, pos - ip[row], ip[row + 1] - ip[row])
             np.testing.assert_array_equal(ns.off[a:b], exp)
             checked += 1
     assert checked > 0 and ns.max_len == int(np.diff(ip)[pat.row_free.astype(bool)].max())
