@@ -130,17 +130,21 @@ __global__ void __launch_bounds__(kSBlock) k_strain(const double *__restrict__ d
 // Stage 2 gathers f_e into rows through the row-slot lists, so every row is written by one
 // lane in a fixed (element) order: deterministic, no atomics.
 // ------------------------------------------------------------------------------------------
-template <int DIM, int NP>
+// NQ > 0: the rule's point count at compile time (the point loop unrolls, so every point's loads
+// are in flight together); NQ == 0: the runtime n_q.
+template <int DIM, int NP, int NQ = 0>
 __global__ void __launch_bounds__(256) k_force_elem(const double *__restrict__ dphi, const double *__restrict__ S,
                                                     const double *__restrict__ w, int64_t n_elem, int64_t n_int,
-                                                    int n_q, double *__restrict__ fe) {
+                                                    int n_q_rt, double *__restrict__ fe) {
     constexpr int NLD = DIM * NP;
     constexpr int NS = BCol<DIM>::NS;
+    const int n_q = NQ > 0 ? NQ : n_q_rt;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_elem * NP) return;
     const int64_t e = t / NP;
     const int node = (int)(t - e * NP);
     double f0 = 0.0, f1 = 0.0, f2 = 0.0;
+#pragma unroll
     for (int q = 0; q < n_q; ++q) {
         const int64_t g = e * n_q + q;
         const double gx = dphi[g * NP + node];
@@ -357,8 +361,12 @@ int launch_strain(const double *dphi, const int32_t *edofs, const double *u, int
 template <int DIM, int NP>
 int launch_force_elem(const double *dphi, const double *S, const double *w, int64_t n_elem, int64_t n_int, int n_q,
                       double *fe, hipStream_t st) {
-    hipLaunchKernelGGL((k_force_elem<DIM, NP>), dim3(ssx::grid_for(n_elem * NP, 256)), dim3(256), 0, st, dphi, S, w,
-                       n_elem, n_int, n_q, fe);
+    if (DIM == 3 && NP == 10 && n_q == 11)
+        hipLaunchKernelGGL((k_force_elem<DIM, NP, 11>), dim3(ssx::grid_for(n_elem * NP, 256)), dim3(256), 0, st, dphi,
+                           S, w, n_elem, n_int, n_q, fe);
+    else
+        hipLaunchKernelGGL((k_force_elem<DIM, NP>), dim3(ssx::grid_for(n_elem * NP, 256)), dim3(256), 0, st, dphi, S,
+                           w, n_elem, n_int, n_q, fe);
     SSX_CHECK_LAUNCH("ssx_force");
     return SSX_OK;
 }
