@@ -73,55 +73,61 @@ __global__ void __launch_bounds__(kSBlock) k_strain(const double *__restrict__ d
                                                     double *__restrict__ E) {
     constexpr int NS = BCol<DIM>::NS;
     constexpr int RS = NP + 1;
-    __shared__ double sg[DIM][kSBlock * RS];
-    __shared__ double se[kSBlock * (NS + 1)];
+    static_assert(DIM * RS >= NS + 1, "the strain staging reuses the gradient area");
+    // gradient rows of the block's points (component c at c*kSBlock*RS), then -- after a barrier --
+    // the block's strains (row stride NS + 1) in the same area: one LDS area keeps more
+    // workgroups resident per CU
+    __shared__ double sg[DIM * kSBlock * RS];
     const int t = threadIdx.x;
     const int64_t g0 = (int64_t)blockIdx.x * kSBlock;
     const int np = (int)(n_int - g0 < kSBlock ? n_int - g0 : kSBlock);
 #pragma unroll
     for (int c = 0; c < DIM; ++c) {
         const double *src = dphi + ((int64_t)c * n_int + g0) * NP;
-        for (int i = t; i < np * NP; i += kSBlock) sg[c][(i / NP) * RS + i % NP] = src[i];
+        for (int i = t; i < np * NP; i += kSBlock) sg[c * kSBlock * RS + (i / NP) * RS + i % NP] = src[i];
     }
     __syncthreads();
+    double a[NS];
+#pragma unroll
+    for (int k = 0; k < NS; ++k) a[k] = 0.0;
     if (t < np) {
         const int64_t g = g0 + t;
         const int64_t e = g / n_q;
         const int32_t *dof = edofs + e * (DIM * NP);
-        const double *gx = sg[0] + t * RS;
-        const double *gy = sg[1] + t * RS;
+        const double *gx = sg + t * RS;
+        const double *gy = sg + kSBlock * RS + t * RS;
         if constexpr (DIM == 3) {
-            const double *gz = sg[2] + t * RS;
-            double a0 = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0, a5 = 0;
+            const double *gz = sg + 2 * kSBlock * RS + t * RS;
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
                 const double ux = u[dof[3 * j]], uy = u[dof[3 * j + 1]], uz = u[dof[3 * j + 2]];
                 const double x = gx[j], y = gy[j], z = gz[j];
-                a0 += x * ux;
-                a1 += y * uy;
-                a2 += z * uz;
-                a3 += y * ux + x * uy;
-                a4 += z * uy + y * uz;
-                a5 += z * ux + x * uz;
+                a[0] += x * ux;
+                a[1] += y * uy;
+                a[2] += z * uz;
+                a[3] += y * ux + x * uy;
+                a[4] += z * uy + y * uz;
+                a[5] += z * ux + x * uz;
             }
-            double *o = se + t * (NS + 1);
-            o[0] = a0; o[1] = a1; o[2] = a2; o[3] = a3; o[4] = a4; o[5] = a5;
         } else {
-            double a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
             for (int j = 0; j < NP; ++j) {
                 const double ux = u[dof[2 * j]], uy = u[dof[2 * j + 1]];
                 const double x = gx[j], y = gy[j];
-                a0 += x * ux;
-                a1 += y * uy;
-                a2 += y * ux + x * uy;
+                a[0] += x * ux;
+                a[1] += y * uy;
+                a[2] += y * ux + x * uy;
             }
-            double *o = se + t * (NS + 1);
-            o[0] = a0; o[1] = a1; o[2] = a2;
         }
     }
+    __syncthreads();   // gradients read: the area now takes the strains
+    if (t < np) {
+        double *o = sg + t * (NS + 1);
+#pragma unroll
+        for (int k = 0; k < NS; ++k) o[k] = a[k];
+    }
     __syncthreads();
-    for (int i = t; i < np * NS; i += kSBlock) E[g0 * NS + i] = se[(i / NS) * (NS + 1) + i % NS];
+    for (int i = t; i < np * NS; i += kSBlock) E[g0 * NS + i] = sg[(i / NS) * (NS + 1) + i % NS];
 }
 
 // ------------------------------------------------------------------------------------------
