@@ -505,6 +505,61 @@ def test_p2_element_blocks_vs_oracle(gpu, fold):
     assert torch.equal(kb[0], kb[1])
 
 
+def _fan_mesh_p2(n_sphere=40, seed=0):
+    """P2 tetrahedra fanned around one vertex: the origin joined to the hull triangles of points on
+    the unit sphere (76 elements share the centre for 40 points), outer vertices 0..9 fixed."""
+    from scipy.spatial import ConvexHull
+    rng = np.random.default_rng(seed)
+    P = rng.normal(size=(n_sphere, 3))
+    P /= np.linalg.norm(P, axis=1)[:, None]
+    tri = ConvexHull(P).simplices
+    coord4 = np.vstack((np.zeros((1, 3)), P)).T
+    elem4 = np.vstack((np.zeros(len(tri), dtype=np.int64), (tri + 1).T))
+    x = coord4[:, elem4]
+    det = np.einsum("ij,ij->j", np.cross(x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], axis=0), x[:, 3] - x[:, 0])
+    neg = det < 0
+    elem4[1, neg], elem4[2, neg] = elem4[2, neg].copy(), elem4[1, neg].copy()
+    coord, elem = mesh.elevate_tet4_to_tet10(coord4, elem4)
+    q = np.ones((3, coord.shape[1]), dtype=bool)
+    q[:, 1:11] = False
+    return coord, elem, q
+
+
+def test_p2_tangent_node_with_more_than_64_elements(gpu):
+    """A node with more element slots than a wavefront has lanes (the block gather reads a node's
+    slot list 64 slots at a time): K_r on a P2 fan mesh (76 elements around the centre) equals the
+    oracle rows kernel; the force gather likewise."""
+    print("case: P2 fan mesh, node with 76 elements", flush=True)
+    coord, elem, q = _fan_mesh_p2()
+    geom = fem.geometry(coord, elem, "P2")
+    pat = pattern.build_owned_pattern(elem, q)
+    dp = DevicePattern(pat, geom, coord=coord)
+    assert int(np.diff(dp.ns.ptr).max()) > 64
+    dev = dp.indptr.device
+    rng = np.random.default_rng(59)
+    A = rng.normal(size=(geom.n_int, 6, 6))
+    DS = (A + A.transpose(0, 2, 1)).reshape(geom.n_int, 36)
+    G = rng.uniform(1e3, 2e4, geom.n_int)
+    lam = rng.uniform(1e3, 3e4, geom.n_int)
+    vol = np.array([1, 1, 1, 0, 0, 0.0])
+    idn = np.diag([1, 1, 1, 0.5, 0.5, 0.5])
+    Del = (lam[:, None, None] * np.outer(vol, vol)[None] + 2 * G[:, None, None] * idn[None]).reshape(-1, 36)
+    r = 0.1
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    dp.tangent(t(DS), K, r=r, mat=(t(G), t(lam)))
+    exp = (1 - r) * oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q) + \
+        r * oracle.tangent_3d_rows(geom.dphi, Del, geom.weight, pat, geom.n_q)
+    exp[pat.constrained_diag_positions] = 1.0
+    assert _rel(K.cpu().numpy(), exp) < 1e-12
+    S = rng.normal(size=(geom.n_int, 6))
+    fe = torch.empty(geom.n_elem * 30, dtype=torch.float64, device=dev)
+    F = torch.empty(pat.n_rows, dtype=torch.float64, device=dev)
+    dp.force(t(S), fe, F)
+    Fo = oracle.force_3d_rows(geom.dphi, S, geom.weight, pat, geom.n_q)
+    assert _rel(F.cpu().numpy(), Fo) < 1e-12
+
+
 def test_p2_tangent_with_branch_ids_vs_oracle(gpu):
     """Tangent from the return mapping's own DS and branch ids (elastic points in closed form) on a
     stratified strain field covering every branch: K_r and K equal the oracle rows kernel on the
