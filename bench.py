@@ -89,6 +89,25 @@ def algorithmic_bytes(op, fold: bool = True, branch: bool = False) -> dict:
     }
 
 
+def moved_bytes(op, fold: bool = True, branch: bool = False) -> dict:
+    """The same kernels' input and output bytes as this build lays them out.  They differ from the
+    algorithmic bytes where the mesh takes the constant-Jacobian form (P2 tetrahedra with straight
+    edges): the strain, force and tangent kernels then read 10 doubles per element (inverse
+    Jacobian and |det J|) in place of the per-point dphi (3 n_p doubles) and w."""
+    ab = algorithmic_bytes(op, fold, branch)
+    pat = op.pattern
+    if not pat.affine:
+        return ab
+    n = op.n_int
+    per_point = n * (24 * pat.n_p + 8)            # dphi + w
+    geo = pat.n_elem * 80
+    mb = dict(ab)
+    for k in ("force", "tangent"):
+        mb[k] = ab[k] - per_point + geo
+    mb["strain"] = ab["strain"] - n * 24 * pat.n_p + geo
+    return mb
+
+
 def run_gpu(args):
     import torch
 
@@ -98,6 +117,7 @@ def run_gpu(args):
     m, op, E_host, br = build_workload(args.h, args.nz, args.unique, args.seed)
     pat = op.pattern
     pat.tangent_method = args.tangent
+    pat.geometry_form = args.geometry
     dev = op._E.device
     S, DS = op._S, op._DS
     K = op.pattern.new_csr(with32=True)
@@ -185,13 +205,18 @@ def run_gpu(args):
         torch.cuda.synchronize()
         del src, dst, rows, idx
     ab = algorithmic_bytes(op, fold=mat is not None, branch=br is not None and mat is not None)
+    mb = moved_bytes(op, fold=mat is not None, branch=br is not None and mat is not None)
     kern = {}
     for name, lst in ev.items():
         ms = float(np.mean([a.elapsed_time(b) for a, b in lst]))
         kern[name] = {"avg_us": round(ms * 1e3, 2), "alg_bytes": int(ab[name]),
                       "GB_s": round(ab[name] / (ms * 1e-3) / 1e9, 1)}
+        if mb[name] != ab[name]:
+            kern[name]["moved_bytes"] = int(mb[name])
+            kern[name]["moved_GB_s"] = round(mb[name] / (ms * 1e-3) / 1e9, 1)
     step_bytes = ab["constitutive"] + ab["force"] + ab["tangent"]
     value = step_bytes / (step_ms * 1e-3) / 1e9
+    moved = (mb["constitutive"] + mb["force"] + mb["tangent"]) / (step_ms * 1e-3) / 1e9
     dom = max(("constitutive", "force", "tangent"), key=lambda k: kern[k]["avg_us"])
     # branch histogram of the inputs as the device classified them
     br_dev = torch.empty(op.n_int, dtype=torch.int8, device=dev)
@@ -207,7 +232,8 @@ def run_gpu(args):
             traffic = None
     return {
         "m": m, "op": op, "E": E_host, "step_ms": step_ms, "value": value, "kernels": kern, "dominant": dom,
-        "wall_s": wall, "hist": hist.tolist(), "traffic": traffic, "ab": ab,
+        "wall_s": wall, "hist": hist.tolist(), "traffic": traffic, "ab": ab, "moved_GB_s": moved,
+        "geometry": "affine" if pat.affine else "points",
     }
 
 
@@ -540,6 +566,9 @@ def main():
     ap.add_argument("--no-probes", action="store_true", help="skip the copy / triad / FP64 rate probes")
     ap.add_argument("--tangent", default="nodes", choices=["nodes", "rows"],
                     help="P2 tangent path: element matrices + node gather (default) or the fused row kernel")
+    ap.add_argument("--geometry", default="affine", choices=["affine", "points"],
+                    help="P2 gradients formed from per-element factors (default, straight-sided meshes) or read "
+                         "per point (dphi, w)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: the path runs on a single GPU, and sharding it over several "
@@ -589,6 +618,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(res["step_ms"], 4),
+        "moved_GB_s": round(res["moved_GB_s"], 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -599,11 +629,12 @@ def main():
                         "(FP64, FP32 shadow)",
             "n_elem": int(op.pattern.n_elem), "n_points": int(op.n_int), "n_dof": int(op.pattern.n_rows),
             "nnz": int(op.pattern.host.nnz), "branch_hist_el_smooth_left_right_apex": res["hist"],
-            "parallelism": "single GPU", "tangent_path": args.tangent,
+            "parallelism": "single GPU", "tangent_path": args.tangent, "geometry": res["geometry"],
         },
         "roofline": {
             "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": res["traffic"],
+            "moved_GB_s": kern[dom].get("moved_GB_s", round(achieved, 1)),
             "traffic_source": ("copied from profiles/traffic.json (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of "
                                "this bench, copy-calibrated; profiles/kernel3d_p2_117k_hbm_traffic.md), not "
                                "measured in this run") if res["traffic"] is not None else None,

@@ -222,6 +222,25 @@ int ssx_gather_node_blocks(int n_p, const double *kb, const int32_t *ns_ptr, con
                            const uint8_t *row_free, const int32_t *nodes, int64_t n_nodes, int max_len, double r,
                            const double *elastic, double *out, float *out32, void *stream);
 
+/* ---- P2 tetrahedra with a constant Jacobian (straight edges, edge nodes at the midpoints) ----
+ * The three entry points below are ssx_strain, ssx_force_elem and ssx_element_blocks_3d for such
+ * meshes, with the per-point dphi and w replaced by per-element factors:
+ *   gm  (n_elem, 10): (J^-1)[d][k] at 3 d + k (the cofactor inverse of fem/assembly.py:139-245,
+ *                     taken at the rule's first point) and |det J| at 9;
+ *   ref (3 n_q n_p + n_q): the rule's reference gradients h_k(q, n) at (k n_q + q) n_p + n, then
+ *                     its weights wf[q];
+ * so that dphi_d(e, q, n) = sum_k gm[e][3 d + k] h_k(q, n) and w(e, q) = gm[e][9] wf[q].  Results
+ * agree with the dphi forms to rounding.  No counterpart in the reference, whose kernels take dphi
+ * and w (cython/assemble_tangent_values_3d.c:13-109, :216-273, :275-327); for callers that own the
+ * mesh and hold it as straight-sided P2 tetrahedra.  n_p 10, n_q 11 only. */
+int ssx_strain_affine_3d(int n_p, const double *gm, const double *ref, const int32_t *edofs, const double *u,
+                         int64_t n_elem, int n_q, double *E, void *stream);
+int ssx_force_elem_affine_3d(int n_p, const double *gm, const double *ref, const double *S, int64_t n_elem, int n_q,
+                             double *fe, void *stream);
+int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, const double *DS, int64_t n_elem,
+                                 int n_q, double r, const double *shear, const double *lame, const int8_t *branch,
+                                 double *kb, void *stream);
+
 /* ---- learned continuation predictor (no reference counterpart: it fills the predictor slot of
  * continuation/indirect.py:2703-2830, where the reference uses the secant of :550-566) ---- */
 

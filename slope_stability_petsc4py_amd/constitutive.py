@@ -23,7 +23,7 @@ import torch
 
 from . import _lib
 from .device import DeviceCSR, DevicePattern, device, ptr, stream_ptr, to_dev
-from .fem import geometry, volume_load
+from .fem import affine_geometry, geometry, volume_load
 from .materials import expand_materials
 from .pattern import build_owned_pattern
 
@@ -132,7 +132,8 @@ class ConstitutiveOperator:
     def from_mesh(cls, mesh, materials, *, Davis_type: str = "B", fp32_shadow: bool = True):
         geom = geometry(mesh.coord, mesh.elem, mesh.elem_type)
         pat = build_owned_pattern(mesh.elem, mesh.q_mask)
-        dpat = DevicePattern(pat, geom, coord=mesh.coord)
+        aff = affine_geometry(mesh.coord, mesh.elem, mesh.elem_type) if mesh.elem_type.upper() == "P2" else None
+        dpat = DevicePattern(pat, geom, coord=mesh.coord, affine=aff)
         c0, phi, psi, shear, bulk, lame, gamma = expand_materials(mesh.material, geom.n_q, materials)
         op = cls(dpat, c0, phi, psi, shear, bulk, lame, Davis_type=Davis_type, q_mask=mesh.q_mask,
                  fp32_shadow=fp32_shadow)

@@ -48,6 +48,13 @@ __host__ __device__ constexpr int voigt(int i, int k) {
 // Start of row a of the upper triangle (pairs (a, a), (a, a+1), ... in row-major order).
 __host__ __device__ constexpr int pair_row_start(int a) { return a * kNP - a * (a - 1) / 2; }
 
+// Value of lane k's v (k uniform), as two 32-bit reads.
+__device__ __forceinline__ double lane_value(double v, int k) {
+    const long long x = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)x, k), hi = __builtin_amdgcn_readlane((int)(x >> 32), k);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 // Lane -> node pair (a, b), a <= b: lanes 0..54 in row-major order over the upper triangle; idle
 // lanes repeat the last pair (their stores are masked).
 __device__ __forceinline__ void lane_pair(int lane, int &p, int &a, int &b) {
@@ -61,7 +68,6 @@ __device__ __forceinline__ void lane_pair(int lane, int &p, int &a, int &b) {
 // Point data of one element staged in LDS: gradients [c][q][node] (330), then w, shear, lame and the
 // elastic flag per point (11 each).
 constexpr int kStG = 3 * kNQ * kNP;
-constexpr int kStWords = kStG + 4 * kNQ;
 constexpr int kStRegs = (kStG + 63) / 64;   // gradient values per lane in flight
 
 // FOLD: shear/lame given, the regularisation r folded in (K_r); EL: per-point branch ids given, and a
@@ -72,12 +78,25 @@ constexpr int kStRegs = (kStG + 63) / 64;   // gradient values per lane in fligh
 // into registers while the current element is computed from LDS and staged into the other LDS
 // buffer afterwards; one load per lane touches every 128-byte line of the next element's D, so
 // the scalar loads of D find it in the L2.
-template <bool FOLD, bool EL>
+//
+// GEO: the element has a constant Jacobian (affine P2 tetrahedron).  In place of its 330 gradient
+// values and 11 weights the next element's 10 factors gm[e] = (J^-1 row-major, |det J|) are
+// fetched (lanes 0..9), and staging forms the gradients from them and the rule's reference
+// gradients h_k(q, n), held in LDS for the kernel's life: g[c][q][n] = sum_k gm[e][3 c + k]
+// h_k(q, n) (six rounds of three FMAs per lane, one component c per round), w_q = gm[e][9] wf[q].
+// The point loop is the same in both forms.
+template <bool FOLD, bool EL, bool GEO>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, const double *__restrict__ w,
                int64_t n_elem, double r, const double *__restrict__ shear, const double *__restrict__ lame,
-               const int8_t *__restrict__ branch, double *__restrict__ kb) {
-    __shared__ double sg[2][kStWords];
+               const int8_t *__restrict__ branch, const double *__restrict__ gm, const double *__restrict__ ref,
+               double *__restrict__ kb) {
+    // per-element staging: gradients [c][q][n], then w, shear, lame and the elastic flag per point
+    constexpr int OFF = kStG;
+    constexpr int NREG = GEO ? 0 : kStRegs;
+    constexpr int NQN = kNQ * kNP;                 // 110 values per component
+    __shared__ double sg[2][OFF + 4 * kNQ];
+    __shared__ double sh[GEO ? kStG + kNQ : 1];   // GEO: h_k(q, n) at k NQN + q kNP + n, then wf
     const int lane = threadIdx.x;
     int p, a, b;
     lane_pair(lane, p, a, b);
@@ -85,41 +104,62 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
     const double om = FOLD ? 1.0 - r : 1.0;
     int64_t e = blockIdx.x;
     if (e >= n_elem) return;   // whole wavefront
+    if constexpr (GEO) {
+        for (int i = lane; i < kStG + kNQ; i += 64) sh[i] = ref[i];
+        __syncthreads();
+    }
 
-    double rg[kStRegs], rw, rs = 0.0, rl = 0.0, touch;
+    double rg[NREG > 0 ? NREG : 1], rw = 0.0, rs = 0.0, rl = 0.0, rgm = 0.0, touch;
     int rb = 1;
     // global -> registers for element ee (every index in bounds; lanes past the data read index 0
     // or the element's last word)
     auto fetch = [&](int64_t ee) {
 #pragma unroll
-        for (int k = 0; k < kStRegs; ++k) {
+        for (int k = 0; k < NREG; ++k) {
             const int t = lane + 64 * k;
             const int tc = t < kStG ? t : 0;
             const int c = tc / (kNQ * kNP), rem = tc - c * (kNQ * kNP);
             rg[k] = dphi[((int64_t)c * n_int + ee * kNQ) * kNP + rem];
         }
         const int lq = lane < kNQ ? lane : 0;
-        rw = w[ee * kNQ + lq];
+        if constexpr (!GEO) rw = w[ee * kNQ + lq];
         if constexpr (FOLD || EL) {
             rs = shear[ee * kNQ + lq];
             rl = lame[ee * kNQ + lq];
         }
         if constexpr (EL) rb = branch[ee * kNQ + lq];
+        if constexpr (GEO) rgm = gm[ee * 10 + (lane < 10 ? lane : 9)];
         touch = DS[ee * (kNQ * 36) + (lane * 16 < kNQ * 36 - 1 ? lane * 16 : kNQ * 36 - 1)];
     };
     // registers -> LDS buffer bb
     auto stage = [&](int bb) {
         asm volatile("" ::"v"(touch));   // the line touch is complete (and kept) before the buffer turns
 #pragma unroll
-        for (int k = 0; k < kStRegs; ++k) {
+        for (int k = 0; k < NREG; ++k) {
             const int t = lane + 64 * k;
             if (t < kStG) sg[bb][t] = rg[k];
         }
+        if constexpr (GEO) {
+            double m[10];
+#pragma unroll
+            for (int k = 0; k < 10; ++k) m[k] = lane_value(rgm, k);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int rem = lane + 64 * h;
+                    if (rem < NQN)
+                        sg[bb][c * NQN + rem] = fma(m[3 * c + 2], sh[2 * NQN + rem],
+                                                    fma(m[3 * c + 1], sh[NQN + rem], m[3 * c] * sh[rem]));
+                }
+            }
+            rw = m[9] * sh[kStG + (lane < kNQ ? lane : 0)];
+        }
         if (lane < kNQ) {
-            sg[bb][kStG + lane] = rw;
-            sg[bb][kStG + kNQ + lane] = rs;
-            sg[bb][kStG + 2 * kNQ + lane] = rl;
-            sg[bb][kStG + 3 * kNQ + lane] = rb == 0 ? 1.0 : 0.0;
+            sg[bb][OFF + lane] = rw;
+            sg[bb][OFF + kNQ + lane] = rs;
+            sg[bb][OFF + 2 * kNQ + lane] = rl;
+            sg[bb][OFF + 3 * kNQ + lane] = rb == 0 ? 1.0 : 0.0;
         }
     };
     fetch(e);
@@ -135,14 +175,16 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
         for (int k = 0; k < kBlk; ++k) acc[k] = 0.0;
 #pragma unroll 1
         for (int q = 0; q < kNQ; ++q) {
-            const double wq = L[kStG + q];
-            const double ha[3] = {L[0 * kNQ * kNP + q * kNP + a], L[1 * kNQ * kNP + q * kNP + a],
-                                  L[2 * kNQ * kNP + q * kNP + a]};
-            const double gb[3] = {L[0 * kNQ * kNP + q * kNP + b], L[1 * kNQ * kNP + q * kNP + b],
-                                  L[2 * kNQ * kNP + q * kNP + b]};
+            const double wq = L[OFF + q];
+            double ha[3], gb[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                ha[d] = L[d * NQN + q * kNP + a];
+                gb[d] = L[d * NQN + q * kNP + b];
+            }
             if constexpr (EL) {
-                if (L[kStG + 3 * kNQ + q] != 0.0) {   // uniform: one point for the whole wavefront
-                    const double wl = wq * L[kStG + 2 * kNQ + q], wg = wq * L[kStG + kNQ + q];
+                if (L[OFF + 3 * kNQ + q] != 0.0) {   // uniform: one point for the whole wavefront
+                    const double wl = wq * L[OFF + 2 * kNQ + q], wg = wq * L[OFF + kNQ + q];
                     const double dg = wg * fma(ha[2], gb[2], fma(ha[1], gb[1], ha[0] * gb[0]));
 #pragma unroll
                     for (int i = 0; i < 3; ++i) {
@@ -172,7 +214,7 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
                     // + w r (B_a^T D_el) row i: lame g_a[i] on the normal strains, 2G g_a[i] more on
                     // strain i, G g_a[k] on the shear strain of (i, k), k != i
                     const double c2 = wq * r;
-                    const double cl = c2 * L[kStG + 2 * kNQ + q], cg = c2 * L[kStG + kNQ + q];
+                    const double cl = c2 * L[OFF + 2 * kNQ + q], cg = c2 * L[OFF + kNQ + q];
                     const double li = cl * ha[i];
 #pragma unroll
                     for (int m = 0; m < 3; ++m) t[m] += li;
@@ -318,6 +360,18 @@ __global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__
 // 4 are resident per SIMD).
 constexpr int kBlkWavesPerCU = 32;
 
+unsigned blocks_grid(int64_t n_elem) {
+    static int n_cu = 0;
+    if (n_cu == 0) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
+            n_cu = 256;
+    }
+    const int64_t want = (int64_t)n_cu * kBlkWavesPerCU;
+    return (unsigned)(n_elem < want ? n_elem : want);
+}
+
 }  // namespace
 
 extern "C" {
@@ -333,23 +387,39 @@ int ssx_element_blocks_3d(int n_p, const double *dphi, const double *DS, const d
     SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_blocks_3d: shear and lame go together");
     SSX_REQUIRE(branch == nullptr || shear != nullptr, "ssx_element_blocks_3d: branch ids need shear and lame");
     hipStream_t st = ssx::as_stream(stream);
-    static int n_cu = 0;
-    if (n_cu == 0) {
-        int dev = 0;
-        if (hipGetDevice(&dev) != hipSuccess ||
-            hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n_cu <= 0)
-            n_cu = 256;
-    }
-    const int64_t want = (int64_t)n_cu * kBlkWavesPerCU;
-    const unsigned grid = (unsigned)(n_elem < want ? n_elem : want);
+    const unsigned grid = blocks_grid(n_elem);
     // FOLD (shear/lame given) applies r; with r = 0 and branch ids the fold terms vanish
-#define KB(F_, E_) \
-    hipLaunchKernelGGL((k_ke_blocks_p2<F_, E_>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, lame, branch, kb)
+#define KB(F_, E_)                                                                                          \
+    hipLaunchKernelGGL((k_ke_blocks_p2<F_, E_, false>), dim3(grid), dim3(64), 0, st, dphi, DS, w, n_elem, r, shear, \
+                       lame, branch, nullptr, nullptr, kb)
     if (branch) KB(true, true);
     else if (shear) KB(true, false);
     else KB(false, false);
 #undef KB
     SSX_CHECK_LAUNCH("ssx_element_blocks_3d");
+    return SSX_OK;
+}
+
+int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, const double *DS, int64_t n_elem,
+                                 int n_q, double r, const double *shear, const double *lame, const int8_t *branch,
+                                 double *kb, void *stream) {
+    SSX_REQUIRE(n_p == kNP && n_q == kNQ, "ssx_element_blocks_affine_3d: only P2 tetrahedra with the 11-point rule "
+                "(n_p 10, n_q 11), got n_p %d n_q %d", n_p, n_q);
+    SSX_REQUIRE(n_elem >= 0 && n_elem <= 0x7fffffff, "ssx_element_blocks_affine_3d: bad element count");
+    if (n_elem == 0) return SSX_OK;
+    SSX_REQUIRE(gm && ref && DS && kb, "ssx_element_blocks_affine_3d: null pointer");
+    SSX_REQUIRE((shear == nullptr) == (lame == nullptr), "ssx_element_blocks_affine_3d: shear and lame go together");
+    SSX_REQUIRE(branch == nullptr || shear != nullptr, "ssx_element_blocks_affine_3d: branch ids need shear and lame");
+    hipStream_t st = ssx::as_stream(stream);
+    const unsigned grid = blocks_grid(n_elem);
+#define KB(F_, E_)                                                                                                  \
+    hipLaunchKernelGGL((k_ke_blocks_p2<F_, E_, true>), dim3(grid), dim3(64), 0, st, nullptr, DS, nullptr, n_elem, r, \
+                       shear, lame, branch, gm, ref, kb)
+    if (branch) KB(true, true);
+    else if (shear) KB(true, false);
+    else KB(false, false);
+#undef KB
+    SSX_CHECK_LAUNCH("ssx_element_blocks_affine_3d");
     return SSX_OK;
 }
 

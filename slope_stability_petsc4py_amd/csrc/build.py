@@ -28,6 +28,7 @@ SOURCES = [
     "mlp.hip",
     "node_assembly.hip",
     "block_assembly.hip",
+    "affine_assembly.hip",
     "probe.hip",
 ]
 

@@ -10,7 +10,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .fem import Geometry
+from .fem import AffineGeometry, Geometry
 from .pattern import OwnedPattern
 
 
@@ -101,7 +101,8 @@ class DeviceCSR:
 class DevicePattern:
     """The fixed pattern, row slots and per-point geometry resident in HBM."""
 
-    def __init__(self, pat: OwnedPattern, geom: Geometry, coord: np.ndarray | None = None):
+    def __init__(self, pat: OwnedPattern, geom: Geometry, coord: np.ndarray | None = None,
+                 affine: AffineGeometry | None = None):
         self.host = pat
         self.geom = geom
         self.dim = pat.dim
@@ -123,6 +124,14 @@ class DevicePattern:
         self.free_idx = to_dev(pat.free_idx)
         self.dphi = to_dev(geom.dphi)
         self.weight = to_dev(geom.weight)
+        # P2 tetrahedra with a constant Jacobian (11-point rule): the strain, force and tangent
+        # kernels form the gradients from 10 numbers per element instead of reading dphi and w
+        self.gm = self.gref = None
+        if affine is not None and self.dim == 3 and self.n_p == 10 and self.n_q == 11:
+            if affine.gm.shape != (self.n_elem, 10) or affine.ref.shape != (3 * self.n_q * self.n_p + self.n_q,):
+                raise ValueError("affine factors do not match the mesh")
+            self.gm = to_dev(affine.gm)
+            self.gref = to_dev(affine.ref)
         self.max_block_nnz = pat.max_block_nnz(64)
         self.lanes = spmv_lanes(pat.indptr)
         self.ns = None
@@ -162,13 +171,28 @@ class DevicePattern:
                          self.lanes)
 
     # -- kernels -----------------------------------------------------------------------------
+    # "affine" (per-element factors, when the mesh has them) or "points" (per-point dphi and w)
+    geometry_form = "affine"
+
+    @property
+    def affine(self) -> bool:
+        return self.gm is not None and self.geometry_form == "affine"
+
     def strain(self, u: torch.Tensor, E: torch.Tensor) -> None:
+        if self.affine:
+            _lib.call("ssx_strain_affine_3d", self.n_p, ptr(self.gm), ptr(self.gref), ptr(self.elem_dofs), ptr(u),
+                      self.n_elem, self.n_q, ptr(E), stream_ptr())
+            return
         _lib.call("ssx_strain", self.dim, self.n_p, ptr(self.dphi), ptr(self.elem_dofs), ptr(u), self.n_elem,
                   self.n_q, ptr(E), stream_ptr())
 
     def force(self, S: torch.Tensor, fe: torch.Tensor, F: torch.Tensor) -> None:
-        _lib.call("ssx_force_elem", self.dim, self.n_p, ptr(self.dphi), ptr(S), ptr(self.weight), self.n_elem,
-                  self.n_q, ptr(fe), stream_ptr())
+        if self.affine:
+            _lib.call("ssx_force_elem_affine_3d", self.n_p, ptr(self.gm), ptr(self.gref), ptr(S), self.n_elem,
+                      self.n_q, ptr(fe), stream_ptr())
+        else:
+            _lib.call("ssx_force_elem", self.dim, self.n_p, ptr(self.dphi), ptr(S), ptr(self.weight), self.n_elem,
+                      self.n_q, ptr(fe), stream_ptr())
         if self.ns is not None and self.force_gather == "nodes":
             _lib.call("ssx_force_gather_nodes", self.dim, self.nld, ptr(fe), ptr(self.ns_ptr), ptr(self.ns_elem),
                       ptr(self.ns_lnode), ptr(self.row_free), self.host.n_nodes, ptr(F), stream_ptr())
@@ -205,8 +229,12 @@ class DevicePattern:
         if self._kb is None:
             self._kb = torch.empty((self.n_elem, 55, 3, 3), dtype=torch.float64, device=DS.device)
         sh, la = (None, None) if mat is None else mat
-        _lib.call("ssx_element_blocks_3d", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
-                  self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
+        if self.affine:
+            _lib.call("ssx_element_blocks_affine_3d", self.n_p, ptr(self.gm), ptr(self.gref), ptr(DS), self.n_elem,
+                      self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
+        else:
+            _lib.call("ssx_element_blocks_3d", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
+                      self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
         return self._kb
 
     def gather_blocks(self, kb: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,

@@ -285,6 +285,55 @@ def geometry(coord: np.ndarray, elem: np.ndarray, elem_type: str, quad: tuple | 
                     dphi=np.ascontiguousarray(dphi), weight=np.ascontiguousarray(weight), hatp=hatp)
 
 
+@dataclass
+class AffineGeometry:
+    """Constant-Jacobian form of a 3D mesh's geometry: with G = gm[e],
+    dphi_d(e, q, n) = sum_k G[3 d + k] h_k(q, n) and w(e, q) = G[9] wf[q]."""
+
+    gm: np.ndarray    # (n_elem, 10): (J^-1)[d, k] at 3 d + k (cofactors / det J), |det J| at 9
+    ref: np.ndarray   # (3 n_q n_p + n_q,): h_k(q, n) at (k n_q + q) n_p + n, then wf
+
+
+def affine_geometry(coord: np.ndarray, elem: np.ndarray, elem_type: str, rtol: float = 1e-10) -> AffineGeometry | None:
+    """Per-element factors of a 3D mesh whose elements are affine images of the reference
+    tetrahedron (straight edges, edge nodes at their midpoints, as the P1 -> P2 elevation places
+    them).  The Jacobian is then the same at every point, so the per-point gradients and weights of
+    ``geometry`` follow from 10 numbers per element and the rule's reference gradients.  The
+    Jacobian is taken at the rule's first point by ``geometry``'s formula (cofactor inverse,
+    fem/assembly.py:139-245); None when some element's Jacobian varies over its points by more than
+    ``rtol`` of its largest entry (curved elements keep the per-point form)."""
+    coord = np.asarray(coord, dtype=np.float64)
+    elem = np.asarray(elem, dtype=np.int64)
+    if coord.shape[0] != 3:
+        return None
+    xi, wf = quadrature_3d(elem_type)
+    _, h1, h2, h3 = basis_3d(elem_type, xi)          # (n_p, n_q) each
+    cx, cy, cz = coord[0, elem].T, coord[1, elem].T, coord[2, elem].T   # (n_e, n_p)
+
+    def jac(q):
+        return [[c @ h[:, q] for c in (cx, cy, cz)] for h in (h1, h2, h3)]   # J[a][b] = dx_b / dxi_a
+
+    J = jac(0)
+    scale = np.max(np.abs(np.array(J)), axis=(0, 1))
+    for q in range(1, xi.shape[1]):
+        Jq = jac(q)
+        dev = np.max(np.abs(np.array(Jq) - np.array(J)), axis=(0, 1))
+        if np.any(dev > rtol * scale):
+            return None
+    (j11, j12, j13), (j21, j22, j23), (j31, j32, j33) = J
+    det = j11 * (j22 * j33 - j23 * j32) - j12 * (j21 * j33 - j23 * j31) + j13 * (j21 * j32 - j22 * j31)
+    inv = 1.0 / det
+    gm = np.empty((elem.shape[1], 10))
+    cof = [(j22 * j33 - j23 * j32), -(j12 * j33 - j13 * j32), (j12 * j23 - j13 * j22),
+           -(j21 * j33 - j23 * j31), (j11 * j33 - j13 * j31), -(j11 * j23 - j13 * j21),
+           (j21 * j32 - j22 * j31), -(j11 * j32 - j12 * j31), (j11 * j22 - j12 * j21)]
+    for k, c in enumerate(cof):
+        gm[:, k] = c * inv
+    gm[:, 9] = np.abs(det)
+    ref = np.concatenate([h1.T.reshape(-1), h2.T.reshape(-1), h3.T.reshape(-1), wf])
+    return AffineGeometry(gm=np.ascontiguousarray(gm), ref=np.ascontiguousarray(ref, dtype=np.float64))
+
+
 def volume_load(geom: Geometry, elem: np.ndarray, n_nodes: int, f_int: np.ndarray) -> np.ndarray:
     """Nodal load f[c, node] = sum_{e,q} hatp[i,q] w_g f_int[c,g] (fem/assembly.py:337-364)."""
     elem = np.asarray(elem, dtype=np.int64)

@@ -79,6 +79,40 @@ def test_volumes_and_weights():
     np.testing.assert_allclose(grad, np.array([[2.0], [-3.0], [0.5]]) * np.ones((1, g3.n_int)), atol=1e-10)
 
 
+def test_affine_geometry_factors():
+    """Per-element factors of straight-sided P2 tetrahedra: for x = A xi + x0 the factors are
+    A^-T and |det A| (textbook inverse Jacobian), they rebuild geometry()'s per-point gradients and
+    weights, and a curved element (one edge node moved off its midpoint) has none."""
+    rng = np.random.default_rng(3)
+    A = rng.normal(size=(3, 3)) + 2 * np.eye(3)
+    x0 = rng.normal(size=3)
+    ref_v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1.0]]).T        # reference vertices
+    coord4 = A @ ref_v + x0[:, None]
+    coord, elem = mesh.elevate_tet4_to_tet10(coord4, np.arange(4)[:, None])
+    aff = fem.affine_geometry(coord, elem, "P2")
+    assert aff is not None
+    np.testing.assert_allclose(aff.gm[0, :9].reshape(3, 3), np.linalg.inv(A).T, rtol=0, atol=1e-13)
+    assert abs(aff.gm[0, 9] - abs(np.linalg.det(A))) < 1e-13
+    # a transposed inverse would not rebuild the gradients: A is not symmetric
+    assert np.abs(np.linalg.inv(A) - np.linalg.inv(A).T).max() > 1e-3
+
+    m = mesh.slope_3d(h=2.5, nz=4)
+    g = fem.geometry(m.coord, m.elem, "P2")
+    aff = fem.affine_geometry(m.coord, m.elem, "P2")
+    n_q, n_p = g.n_q, g.n_p
+    H = aff.ref[:3 * n_q * n_p].reshape(3, n_q, n_p)
+    wf = aff.ref[3 * n_q * n_p:]
+    M = aff.gm[:, :9].reshape(-1, 3, 3)
+    dphi = np.einsum("edk,kqn->deqn", M, H).reshape(3, -1, n_p)
+    w = (aff.gm[:, 9:10] * wf[None]).reshape(-1)
+    assert np.abs(dphi - g.dphi).max() < 1e-13 * np.abs(g.dphi).max()
+    assert np.abs(w - g.weight).max() < 1e-13 * g.weight.max()
+    bent = m.coord.copy()
+    bent[:, m.elem[4, 7]] += 0.05 * np.array([1.0, 0.5, -0.25])
+    assert fem.affine_geometry(bent, m.elem, "P2") is None
+    assert fem.affine_geometry(m.coord, m.elem, "P2", rtol=1e-10) is not None
+
+
 def test_tet10_elevation_first_encounter_order():
     coord = np.array([[0, 1, 0, 0, 1], [0, 0, 1, 0, 1], [0, 0, 0, 1, 1]], dtype=float)
     elem = np.array([[0, 1], [1, 2], [2, 3], [3, 4]])

@@ -601,6 +601,80 @@ def test_p2_tangent_with_branch_ids_vs_oracle(gpu):
         dp.tangent(DSd, K2, r=0.1, branch=br)
 
 
+@pytest.mark.parametrize("kind", ["slope", "fan"])
+def test_p2_affine_geometry_kernels_vs_oracle(gpu, kind):
+    """Strain, force and tangent of straight-sided P2 tetrahedra with the gradients formed in the
+    kernels from the per-element factors: equal to the oracle on the per-point dphi and w to 1e-12,
+    to the per-point kernels to 1e-13, and the tangent repeats bit for bit (a slope mesh whose
+    point count is not a multiple of the strain block, and the fan mesh of 76 elements)."""
+    print(f"case: P2 affine geometry kernels, {kind}", flush=True)
+    if kind == "slope":
+        m = mesh.slope_3d(h=2.5, x1=10.0, x2=10.0, x3=10.0, y1=10.0, y2=10.0, z_len=10.0, nz=3)
+        coord, elem, q = m.coord, m.elem, m.q_mask
+    else:
+        coord, elem, q = _fan_mesh_p2()
+    geom = fem.geometry(coord, elem, "P2")
+    pat = pattern.build_owned_pattern(elem, q)
+    aff = fem.affine_geometry(coord, elem, "P2")
+    assert aff is not None
+    dp = DevicePattern(pat, geom, coord=coord, affine=aff)
+    pts = DevicePattern(pat, geom, coord=coord)
+    assert dp.affine and not pts.affine
+    dev = dp.indptr.device
+    n = geom.n_int
+    if kind == "slope":
+        assert n % 256 != 0
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)  # noqa: E731
+    rng = np.random.default_rng(61)
+    u = rng.normal(0, 1e-3, 3 * coord.shape[1])
+    E = torch.empty((n, 6), dtype=torch.float64, device=dev)
+    E2 = torch.empty_like(E)
+    dp.strain(t(u), E)
+    pts.strain(t(u), E2)
+    assert _rel(E.cpu().numpy(), oracle.strain_3d(geom.dphi, u, pat.elem_dofs, geom.n_q)) < 1e-12
+    assert _rel(E.cpu().numpy(), E2.cpu().numpy()) < 1e-13
+    S = rng.normal(size=(n, 6))
+    fe = torch.empty(geom.n_elem * 30, dtype=torch.float64, device=dev)
+    F = torch.empty(pat.n_rows, dtype=torch.float64, device=dev)
+    F2 = torch.empty_like(F)
+    dp.force(t(S), fe, F)
+    pts.force(t(S), fe, F2)
+    assert _rel(F.cpu().numpy(), oracle.force_3d_rows(geom.dphi, S, geom.weight, pat, geom.n_q)) < 1e-12
+    assert _rel(F.cpu().numpy(), F2.cpu().numpy()) < 1e-13
+    # tangent from the return mapping's DS and branch ids, K_r folded
+    Es, _b = W.stratified_strains(n, seed=8)
+    cb, sp = np.full(n, 4 * np.sqrt(3)), np.full(n, 1 / np.sqrt(3))
+    G, lam = np.full(n, W.HOMO.shear), np.full(n, W.HOMO.lame)
+    _S, DSd, br = constitutive_problem_3d(Es, cb, sp, G, lam, return_tangent=True, return_branch=True)
+    DS = DSd.cpu().numpy()
+    vol = np.array([1, 1, 1, 0, 0, 0.0])
+    idn = np.diag([1, 1, 1, 0.5, 0.5, 0.5])
+    Del = (lam[:, None, None] * np.outer(vol, vol)[None] + 2 * G[:, None, None] * idn[None]).reshape(-1, 36)
+    Kt = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
+    Kel = oracle.tangent_3d_rows(geom.dphi, Del, geom.weight, pat, geom.n_q)
+    for r in (0.0, 0.25):
+        exp = (1 - r) * Kt + r * Kel
+        exp[pat.constrained_diag_positions] = 1.0
+        K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+        K32 = torch.empty(pat.nnz, dtype=torch.float32, device=dev)
+        dp.tangent(DSd, K, K32, r=r, mat=(t(G), t(lam)), branch=br)
+        assert _rel(K.cpu().numpy(), exp) < 1e-12, r
+        np.testing.assert_array_equal(K32.cpu().numpy(), K.cpu().numpy().astype(np.float32))
+        K2 = torch.empty_like(K)
+        dp.tangent(DSd, K2, r=r, mat=(t(G), t(lam)), branch=br)
+        assert torch.equal(K, K2)
+        pts.tangent(DSd, K2, r=r, mat=(t(G), t(lam)), branch=br)
+        assert _rel(K.cpu().numpy(), K2.cpu().numpy()) < 1e-13
+        # without branch ids and without the fold
+        dp.tangent(DSd, K2, r=r, mat=(t(G), t(lam)))
+        assert _rel(K.cpu().numpy(), K2.cpu().numpy()) < 1e-13
+    K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
+    dp.tangent(DSd, K)
+    exp = Kt.copy()
+    exp[pat.constrained_diag_positions] = 1.0
+    assert _rel(K.cpu().numpy(), exp) < 1e-12
+
+
 @pytest.mark.parametrize("n", [1, 64, 200, 1000])
 @pytest.mark.parametrize("f64", [False, True])
 def test_sym_tiles_gemv_vs_numpy(gpu, n, f64):
