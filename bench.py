@@ -89,11 +89,12 @@ def algorithmic_bytes(op, fold: bool = True, branch: bool = False) -> dict:
     }
 
 
-def moved_bytes(op, fold: bool = True, branch: bool = False) -> dict:
+def moved_bytes(op, fold: bool = True, branch: bool = False, sym: bool = False) -> dict:
     """The same kernels' input and output bytes as this build lays them out.  They differ from the
     algorithmic bytes where the mesh takes the constant-Jacobian form (P2 tetrahedra with straight
     edges): the strain, force and tangent kernels then read 10 doubles per element (inverse
-    Jacobian and |det J|) in place of the per-point dphi (3 n_p doubles) and w."""
+    Jacobian and |det J|) in place of the per-point dphi (3 n_p doubles) and w, and (`sym`) the
+    return mapping writes, and the tangent reads, the symmetric DS as 21 values per point, not 36."""
     ab = algorithmic_bytes(op, fold, branch)
     pat = op.pattern
     if not pat.affine:
@@ -105,6 +106,9 @@ def moved_bytes(op, fold: bool = True, branch: bool = False) -> dict:
     for k in ("force", "tangent"):
         mb[k] = ab[k] - per_point + geo
     mb["strain"] = ab["strain"] - n * 24 * pat.n_p + geo
+    if sym:
+        mb["constitutive"] -= n * 15 * 8
+        mb["tangent"] -= n * 15 * 8
     return mb
 
 
@@ -119,7 +123,12 @@ def run_gpu(args):
     pat.tangent_method = args.tangent
     pat.geometry_form = args.geometry
     dev = op._E.device
-    S, DS = op._S, op._DS
+    # P2 tetrahedra in the constant-Jacobian form: the return mapping writes the symmetric tangent
+    # packed (21 values per point) and the tangent reads it so
+    sym = bool(op._sym and pat.affine and args.tangent == "nodes")
+    S = op._S
+    DS = op._DSp if sym else (op._DS if op._DS is not None else
+                              torch.empty((op.n_int, 36), dtype=torch.float64, device=dev))
     K = op.pattern.new_csr(with32=True)
     fe, F = op._fe, op._F
     st = stream_ptr()
@@ -129,8 +138,12 @@ def run_gpu(args):
     br = op._branch if args.tangent == "nodes" else None
 
     def k_const():
-        _lib.call("ssx_constitutive_3d", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear), ptr(op.lame),
-                  op.n_int, ptr(S), ptr(DS), None, ptr(br), st)
+        if sym:
+            _lib.call("ssx_constitutive_3d_sym", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear),
+                      ptr(op.lame), op.n_int, ptr(S), ptr(DS), ptr(br), st)
+        else:
+            _lib.call("ssx_constitutive_3d", ptr(op._E), ptr(op.c_bar), ptr(op.sin_phi), ptr(op.shear),
+                      ptr(op.lame), op.n_int, ptr(S), ptr(DS), None, ptr(br), st)
 
     def k_force():
         pat.force(S, fe, F)
@@ -139,7 +152,8 @@ def run_gpu(args):
     el = None if mat is not None else op.elastic_values
 
     def k_tangent():
-        pat.tangent(DS, K.values, K.values32, r=r, elastic=el, mat=mat, branch=br if mat is not None else None)
+        pat.tangent(DS, K.values, K.values32, r=r, elastic=el, mat=mat, branch=br if mat is not None else None,
+                    sym=sym)
 
     u = torch.from_numpy(np.random.default_rng(1).normal(0, 1e-3, pat.n_rows)).to(dev)
     E_tmp = torch.empty_like(op._E)
@@ -205,7 +219,7 @@ def run_gpu(args):
         torch.cuda.synchronize()
         del src, dst, rows, idx
     ab = algorithmic_bytes(op, fold=mat is not None, branch=br is not None and mat is not None)
-    mb = moved_bytes(op, fold=mat is not None, branch=br is not None and mat is not None)
+    mb = moved_bytes(op, fold=mat is not None, branch=br is not None and mat is not None, sym=sym)
     kern = {}
     for name, lst in ev.items():
         ms = float(np.mean([a.elapsed_time(b) for a, b in lst]))
@@ -233,7 +247,7 @@ def run_gpu(args):
     return {
         "m": m, "op": op, "E": E_host, "step_ms": step_ms, "value": value, "kernels": kern, "dominant": dom,
         "wall_s": wall, "hist": hist.tolist(), "traffic": traffic, "ab": ab, "moved_GB_s": moved,
-        "geometry": "affine" if pat.affine else "points",
+        "geometry": "affine" if pat.affine else "points", "ds_layout": "sym21" if sym else "full36",
     }
 
 
@@ -380,6 +394,7 @@ def run_ssr_leg(log_progress=True) -> dict:
         res = cases.run_ssr(s, predictor=pred)
         out[name] = {"wall_s": round(res.wall_time, 3), "fos": res.fos, "fos_abs_err": abs(res.fos - fos_ref),
                      "newton": res.newton_iterations, "linear": res.linear_iterations,
+                     "ms_per_linear_iteration": round(1e3 * res.wall_time / max(res.linear_iterations, 1), 3),
                      "learned_steps": sum(1 for r in res.step_info if r["predictor"] == "learned")}
         finals[name] = (res.U.clone(), float(res.fos), float(res.omega_hist[-1]))
         if name == "plain_fp64":
@@ -630,6 +645,7 @@ def main():
             "n_elem": int(op.pattern.n_elem), "n_points": int(op.n_int), "n_dof": int(op.pattern.n_rows),
             "nnz": int(op.pattern.host.nnz), "branch_hist_el_smooth_left_right_apex": res["hist"],
             "parallelism": "single GPU", "tangent_path": args.tangent, "geometry": res["geometry"],
+                       "ds_layout": res["ds_layout"],
         },
         "roofline": {
             "kernel": dom, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

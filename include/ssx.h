@@ -53,6 +53,17 @@ int ssx_constitutive_3d(const double *E, const double *c_bar, const double *sin_
 int ssx_constitutive_3d_mat(const double *E, const int32_t *mat_id, const double *table, int n_mat, int n_q,
                             int64_t n, double *S, double *DS, float *DS32, int8_t *branch, void *stream);
 
+/* ssx_constitutive_3d with the tangent written packed: DSp (n x 21) holds DS[s + 6 m] for s <= m,
+ * row by row (s = 0: m = 0..5; s = 1: m = 1..5; ...), the same values as the full layout's upper
+ * entries.  The associated-flow tangent is symmetric (sin psi = sin phi in the reference kernel,
+ * cython/constitutive_3D_kernel.h:73), so the 21 entries carry it whole; no counterpart in the
+ * reference, for callers that hand DS straight to ssx_element_blocks_affine_3d (ds_sym = 1). */
+int ssx_constitutive_3d_sym(const double *E, const double *c_bar, const double *sin_phi, const double *shear,
+                            const double *lame, int64_t n, double *S, double *DSp, int8_t *branch, void *stream);
+
+/* The full point-major DS (n x 36) of a packed one: DS[s + 6 m] = DSp[(min, max) of (s, m)]. */
+int ssx_ds_unpack_sym(int64_t n, const double *DSp, double *DS, void *stream);
+
 /* Plane-strain return mapping: E (n x 3) -> S (n x 4), DS (n x 9).
  * Replaces constitutive_problem_2D (constitutive/problem.py:143-475). */
 int ssx_constitutive_2d(const double *E, const double *c_bar, const double *sin_phi, const double *shear,
@@ -237,9 +248,10 @@ int ssx_strain_affine_3d(int n_p, const double *gm, const double *ref, const int
                          int64_t n_elem, int n_q, double *E, void *stream);
 int ssx_force_elem_affine_3d(int n_p, const double *gm, const double *ref, const double *S, int64_t n_elem, int n_q,
                              double *fe, void *stream);
-int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, const double *DS, int64_t n_elem,
-                                 int n_q, double r, const double *shear, const double *lame, const int8_t *branch,
-                                 double *kb, void *stream);
+/* ds_sym = 1: DS is the packed (n_int x 21) tangent of ssx_constitutive_3d_sym. */
+int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, const double *DS, int ds_sym,
+                                 int64_t n_elem, int n_q, double r, const double *shear, const double *lame,
+                                 const int8_t *branch, double *kb, void *stream);
 
 /* ---- learned continuation predictor (no reference counterpart: it fills the predictor slot of
  * continuation/indirect.py:2703-2830, where the reference uses the secant of :550-566) ---- */

@@ -60,7 +60,9 @@ _SIGS: dict[str, list] = {
     "ssx_probe_fp64": [I32, I64, I64, P, P, P],
     "ssx_gather_node_rows": [I32, I32, P, P, P, P, P, P, P, P, I64, I32, F64, P, P, P, P],
     "ssx_element_blocks_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P, P],
-    "ssx_element_blocks_affine_3d": [I32, P, P, P, I64, I32, F64, P, P, P, P, P],
+    "ssx_element_blocks_affine_3d": [I32, P, P, P, I32, I64, I32, F64, P, P, P, P, P],
+    "ssx_constitutive_3d_sym": [P, P, P, P, P, I64, P, P, P, P],
+    "ssx_ds_unpack_sym": [I64, P, P, P],
     "ssx_strain_affine_3d": [I32, P, P, P, P, I64, I32, P, P],
     "ssx_force_elem_affine_3d": [I32, P, P, P, I64, I32, P, P],
     "ssx_gather_node_blocks": [I32, P, P, P, P, P, P, P, P, I64, I32, F64, P, P, P, P],

@@ -111,11 +111,16 @@ class ConstitutiveOperator:
         ns = self.n_strain
         self._E = torch.empty((self.n_int, ns), dtype=torch.float64, device=dev)
         self._S = torch.empty((self.n_int, ns + (1 if self.dim == 2 else 0)), dtype=torch.float64, device=dev)
-        self._DS = torch.empty((self.n_int, ns * ns), dtype=torch.float64, device=dev)
-        # branch ids of the return mapping that wrote _DS (P2 tetrahedra: elastic points then enter
-        # the tangent in closed form)
+        # branch ids of the return mapping that wrote the tangent (P2 tetrahedra: elastic points then
+        # enter the tangent in closed form)
         self._branch = (torch.empty(self.n_int, dtype=torch.int8, device=dev)
                         if self.dim == 3 and pattern.n_p == 10 and pattern.ns is not None else None)
+        # P2 tetrahedra in the constant-Jacobian form keep the (symmetric) tangent packed, 21 values
+        # per point (_DSp); the full (n_int, 36) array is unpacked only when DS is asked for
+        self._sym = self._branch is not None and pattern.affine
+        self._DSp = torch.empty((self.n_int, 21), dtype=torch.float64, device=dev) if self._sym else None
+        self._DS = None if self._sym else torch.empty((self.n_int, ns * ns), dtype=torch.float64, device=dev)
+        self._ds_unpacked = False
         self._fe = torch.empty(pattern.n_elem * pattern.nld, dtype=torch.float64, device=dev)
         self._F = torch.empty(pattern.n_rows, dtype=torch.float64, device=dev)
         self._ds_ready = False
@@ -212,7 +217,12 @@ class ConstitutiveOperator:
         t1 = perf_counter()
         DS = self._DS if tangent else None
         br = self._branch if tangent else None
-        if self.dim == 3:
+        if self.dim == 3 and self._sym:
+            _lib.call("ssx_constitutive_3d_sym", ptr(self._E), ptr(self.c_bar), ptr(self.sin_phi), ptr(self.shear),
+                      ptr(self.lame), self.n_int, ptr(self._S), ptr(self._DSp if tangent else None), ptr(br),
+                      stream_ptr())
+            self._ds_unpacked = False
+        elif self.dim == 3:
             _lib.call("ssx_constitutive_3d", ptr(self._E), ptr(self.c_bar), ptr(self.sin_phi), ptr(self.shear),
                       ptr(self.lame), self.n_int, ptr(self._S), ptr(DS), None, ptr(br), stream_ptr())
         else:
@@ -231,7 +241,19 @@ class ConstitutiveOperator:
 
     @property
     def DS(self):
-        return self._DS.t() if self._ds_ready else None
+        """Consistent tangent as (n_strain^2, n_int), the reference's orientation (a transposed view)."""
+        if not self._ds_ready:
+            return None
+        if self._sym and not self._ds_unpacked:
+            if self._DS is None:
+                self._DS = torch.empty((self.n_int, 36), dtype=torch.float64, device=self._DSp.device)
+            _lib.call("ssx_ds_unpack_sym", self.n_int, ptr(self._DSp), ptr(self._DS), stream_ptr())
+            self._ds_unpacked = True
+        return self._DS.t()
+
+    def _tangent_input(self):
+        """(DS as the tangent kernels take it, packed?)"""
+        return (self._DSp, True) if self._sym else (self._DS, False)
 
     @property
     def E(self):
@@ -297,8 +319,9 @@ class ConstitutiveOperator:
         t0 = perf_counter()
         if self._branch is not None:
             # K = K_r at r = 0, with the elastic points in closed form
-            self.pattern.tangent(self._DS, self._owned_tangent_mat.values, r=0.0, mat=(self.shear, self.lame),
-                                 branch=self._branch)
+            DS, sym = self._tangent_input()
+            self.pattern.tangent(DS, self._owned_tangent_mat.values, r=0.0, mat=(self.shear, self.lame),
+                                 branch=self._branch, sym=sym)
         else:
             self.pattern.tangent(self._DS, self._owned_tangent_mat.values)
         self.time_build_tangent_local.append(perf_counter() - t0)
@@ -321,8 +344,9 @@ class ConstitutiveOperator:
         t0 = perf_counter()
         if self.pattern.ns is not None:
             # K_r folded into the point tangents: (1-r) DS + r D_el (same operator, no K_el read)
-            self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), mat=(self.shear, self.lame),
-                                 branch=self._branch)
+            DS, sym = self._tangent_input()
+            self.pattern.tangent(DS, K.values, K.values32, r=float(r), mat=(self.shear, self.lame),
+                                 branch=self._branch, sym=sym)
         else:
             self.pattern.tangent(self._DS, K.values, K.values32, r=float(r), elastic=self.elastic_values)
         self.time_build_tangent_local.append(perf_counter() - t0)

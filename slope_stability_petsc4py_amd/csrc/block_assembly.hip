@@ -25,7 +25,8 @@
 // through t (D_el has the closed form lame m m^T + 2G on the normal and G on the shear diagonal).
 //
 // Summation order of a CSR entry: element blocks over the node's elements in element order (the
-// order of the reference's rows kernel), each block summed over the points in order; bitwise
+// order of the reference's rows kernel), each block summed over the points in order (with branch
+// ids: the points the return mapping left elastic in order, then the others in order); bitwise
 // reproducible.
 //
 // Reference behaviour: cython/assemble_tangent_values_3d.c:13-109 (element matrices), :111-214
@@ -47,6 +48,13 @@ __host__ __device__ constexpr int voigt(int i, int k) {
 
 // Start of row a of the upper triangle (pairs (a, a), (a, a+1), ... in row-major order).
 __host__ __device__ constexpr int pair_row_start(int a) { return a * kNP - a * (a - 1) / 2; }
+
+// Word of D[s][m] in a point's tangent: s + 6 m (all 36), or the packed upper entry of (min, max).
+template <bool SYM>
+__host__ __device__ constexpr int ds_at(int s, int m) {
+    return !SYM ? s + 6 * m
+                : (s <= m ? s * 6 - s * (s - 1) / 2 + (m - s) : m * 6 - m * (m - 1) / 2 + (s - m));
+}
 
 // Value of lane k's v (k uniform), as two 32-bit reads.
 __device__ __forceinline__ double lane_value(double v, int k) {
@@ -85,13 +93,17 @@ constexpr int kStRegs = (kStG + 63) / 64;   // gradient values per lane in fligh
 // gradients h_k(q, n), held in LDS for the kernel's life: g[c][q][n] = sum_k gm[e][3 c + k]
 // h_k(q, n) (six rounds of three FMAs per lane, one component c per round), w_q = gm[e][9] wf[q].
 // The point loop is the same in both forms.
-template <bool FOLD, bool EL, bool GEO>
+// SYM: DS holds the 21 entries D[s][m], s <= m, of each point (packed row by row) instead of 36; the
+// lower entries are read from their mirror (the tangent is symmetric), so the point loop is the
+// same with other constant offsets.
+template <bool FOLD, bool EL, bool GEO, bool SYM = false>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, const double *__restrict__ w,
                int64_t n_elem, double r, const double *__restrict__ shear, const double *__restrict__ lame,
                const int8_t *__restrict__ branch, const double *__restrict__ gm, const double *__restrict__ ref,
                double *__restrict__ kb) {
     // per-element staging: gradients [c][q][n], then w, shear, lame and the elastic flag per point
+    constexpr int DW = SYM ? 21 : 36;              // words of a point's D
     constexpr int OFF = kStG;
     constexpr int NREG = GEO ? 0 : kStRegs;
     constexpr int NQN = kNQ * kNP;                 // 110 values per component
@@ -129,7 +141,7 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
         }
         if constexpr (EL) rb = branch[ee * kNQ + lq];
         if constexpr (GEO) rgm = gm[ee * 10 + (lane < 10 ? lane : 9)];
-        touch = DS[ee * (kNQ * 36) + (lane * 16 < kNQ * 36 - 1 ? lane * 16 : kNQ * 36 - 1)];
+        touch = DS[ee * (kNQ * DW) + (lane * 16 < kNQ * DW - 1 ? lane * 16 : kNQ * DW - 1)];
     };
     // registers -> LDS buffer bb
     auto stage = [&](int bb) {
@@ -173,30 +185,44 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
         double acc[kBlk];
 #pragma unroll
         for (int k = 0; k < kBlk; ++k) acc[k] = 0.0;
-#pragma unroll 1
-        for (int q = 0; q < kNQ; ++q) {
-            const double wq = L[OFF + q];
-            double ha[3], gb[3];
+        auto grads = [&](int q, double ha[3], double gb[3]) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 ha[d] = L[d * NQN + q * kNP + a];
                 gb[d] = L[d * NQN + q * kNP + b];
             }
-            if constexpr (EL) {
-                if (L[OFF + 3 * kNQ + q] != 0.0) {   // uniform: one point for the whole wavefront
-                    const double wl = wq * L[OFF + 2 * kNQ + q], wg = wq * L[OFF + kNQ + q];
-                    const double dg = wg * fma(ha[2], gb[2], fma(ha[1], gb[1], ha[0] * gb[0]));
+        };
+        // Two point loops, each with one arithmetic path and a uniform skip: the points the return
+        // mapping left elastic (closed form), then the others.  (One loop with both paths makes the
+        // compiler keep the two paths' accumulators in different registers and copy all nine
+        // between them at the join and on the back edge, 18 moves per point.)
+        if constexpr (EL) {
+#pragma unroll 1
+            for (int q = 0; q < kNQ; ++q) {
+                if (L[OFF + 3 * kNQ + q] == 0.0) continue;   // uniform: one point for the whole wavefront
+                const double wq = L[OFF + q];
+                double ha[3], gb[3];
+                grads(q, ha, gb);
+                const double wl = wq * L[OFF + 2 * kNQ + q], wg = wq * L[OFF + kNQ + q];
+                const double dg = wg * fma(ha[2], gb[2], fma(ha[1], gb[1], ha[0] * gb[0]));
 #pragma unroll
-                    for (int i = 0; i < 3; ++i) {
-                        const double li = wl * ha[i], gi = wg * gb[i];
+                for (int i = 0; i < 3; ++i) {
+                    const double li = wl * ha[i], gi = wg * gb[i];
 #pragma unroll
-                        for (int j = 0; j < 3; ++j) acc[3 * i + j] = fma(gi, ha[j], fma(li, gb[j], acc[3 * i + j]));
-                        acc[4 * i] += dg;
-                    }
-                    continue;
+                    for (int j = 0; j < 3; ++j) acc[3 * i + j] = fma(gi, ha[j], fma(li, gb[j], acc[3 * i + j]));
+                    acc[4 * i] += dg;
                 }
             }
-            const double *D = DS + (e * kNQ + q) * 36;   // D[s][m] = D[s + 6 m]; uniform: scalar loads
+        }
+#pragma unroll 1
+        for (int q = 0; q < kNQ; ++q) {
+            if constexpr (EL) {
+                if (L[OFF + 3 * kNQ + q] != 0.0) continue;   // uniform: taken in the first loop
+            }
+            const double wq = L[OFF + q];
+            double ha[3], gb[3];
+            grads(q, ha, gb);
+            const double *D = DS + (e * kNQ + q) * DW;   // D[s][m] = D[ds_at(s, m)]; uniform: scalar loads
             const double c1 = wq * om;
             const double ga[3] = {c1 * ha[0], c1 * ha[1], c1 * ha[2]};
 #pragma unroll
@@ -205,9 +231,9 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
                 double t[6];
 #pragma unroll
                 for (int m = 0; m < 6; ++m) {
-                    double v = ga[0] * D[voigt(i, 0) + 6 * m];
-                    v = fma(ga[1], D[voigt(i, 1) + 6 * m], v);
-                    v = fma(ga[2], D[voigt(i, 2) + 6 * m], v);
+                    double v = ga[0] * D[ds_at<SYM>(voigt(i, 0), m)];
+                    v = fma(ga[1], D[ds_at<SYM>(voigt(i, 1), m)], v);
+                    v = fma(ga[2], D[ds_at<SYM>(voigt(i, 2), m)], v);
                     t[m] = v;
                 }
                 if constexpr (FOLD) {
@@ -233,12 +259,15 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
                 }
             }
         }
+        // The next element is staged before this element's blocks are stored: its loads were
+        // issued before the point loops, and the wait for them (in issue order) then does not also
+        // wait for the stores.
+        if (en < n_elem) stage(buf ^ 1);
         if (lane < kNPair) {
             double *o = kb + (e * kNPair + p) * kBlk;
 #pragma unroll
             for (int k = 0; k < kBlk; ++k) o[k] = acc[k];
         }
-        if (en < n_elem) stage(buf ^ 1);
         __syncthreads();
         buf ^= 1;
     }
@@ -356,9 +385,11 @@ __global__ void __launch_bounds__(64) k_gather_blocks(const double *__restrict__
     }
 }
 
-// Persistent grid of the block kernel: 32 wavefronts per CU (measured best of 12-32 on MI355X;
-// 4 are resident per SIMD).
-constexpr int kBlkWavesPerCU = 32;
+// Persistent grid of the block kernel: 64 wavefronts per CU, about 3.5 times the resident count
+// (LDS holds 18 per CU).  Measured on MI355X on the 117k-element bench mesh: 300 us at 32, 270-275
+// us at 64-128, 285 us with one element per wavefront; later wavefronts start as earlier ones end,
+// which keeps the CUs' memory and arithmetic phases from running in step.
+constexpr int kBlkWavesPerCU = 64;
 
 unsigned blocks_grid(int64_t n_elem) {
     static int n_cu = 0;
@@ -400,9 +431,9 @@ int ssx_element_blocks_3d(int n_p, const double *dphi, const double *DS, const d
     return SSX_OK;
 }
 
-int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, const double *DS, int64_t n_elem,
-                                 int n_q, double r, const double *shear, const double *lame, const int8_t *branch,
-                                 double *kb, void *stream) {
+int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, const double *DS, int ds_sym,
+                                 int64_t n_elem, int n_q, double r, const double *shear, const double *lame,
+                                 const int8_t *branch, double *kb, void *stream) {
     SSX_REQUIRE(n_p == kNP && n_q == kNQ, "ssx_element_blocks_affine_3d: only P2 tetrahedra with the 11-point rule "
                 "(n_p 10, n_q 11), got n_p %d n_q %d", n_p, n_q);
     SSX_REQUIRE(n_elem >= 0 && n_elem <= 0x7fffffff, "ssx_element_blocks_affine_3d: bad element count");
@@ -412,12 +443,18 @@ int ssx_element_blocks_affine_3d(int n_p, const double *gm, const double *ref, c
     SSX_REQUIRE(branch == nullptr || shear != nullptr, "ssx_element_blocks_affine_3d: branch ids need shear and lame");
     hipStream_t st = ssx::as_stream(stream);
     const unsigned grid = blocks_grid(n_elem);
-#define KB(F_, E_)                                                                                                  \
-    hipLaunchKernelGGL((k_ke_blocks_p2<F_, E_, true>), dim3(grid), dim3(64), 0, st, nullptr, DS, nullptr, n_elem, r, \
-                       shear, lame, branch, gm, ref, kb)
-    if (branch) KB(true, true);
-    else if (shear) KB(true, false);
-    else KB(false, false);
+#define KB(F_, E_, S_)                                                                                   \
+    hipLaunchKernelGGL((k_ke_blocks_p2<F_, E_, true, S_>), dim3(grid), dim3(64), 0, st, nullptr, DS, nullptr, \
+                       n_elem, r, shear, lame, branch, gm, ref, kb)
+    if (ds_sym) {
+        if (branch) KB(true, true, true);
+        else if (shear) KB(true, false, true);
+        else KB(false, false, true);
+    } else {
+        if (branch) KB(true, true, false);
+        else if (shear) KB(true, false, false);
+        else KB(false, false, false);
+    }
 #undef KB
     SSX_CHECK_LAUNCH("ssx_element_blocks_affine_3d");
     return SSX_OK;

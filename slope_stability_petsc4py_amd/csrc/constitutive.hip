@@ -59,10 +59,20 @@ __device__ __forceinline__ void eig_proj(const double esq[6], const double et[6]
     for (int i = 0; i < 6; ++i) P[i] = inv * (esq[i] - apb * et[i] + (i < 3 ? atb : 0.0));
 }
 
+// Packed symmetric layout of a 6x6 tangent: the 21 entries DS[s + 6 m], s <= m, row by row
+// (s = 0: m = 0..5, s = 1: m = 1..5, ...).  sym_full(k) = s + 6 m of packed entry k (integer
+// arithmetic: row s starts at 6 s - s (s - 1) / 2).
+__device__ __forceinline__ int sym_full(int k) {
+    const int s = (k >= 6) + (k >= 11) + (k >= 15) + (k >= 18) + (k >= 20);
+    const int m = s + (k - (6 * s - s * (s - 1) / 2));
+    return s + 6 * m;
+}
+
 struct DSWriter {
-    double *ds;       // point-major DS (n x 36) or nullptr
+    double *ds;       // point-major DS (n x 36; n x 21 packed when sym) or nullptr
     float *ds32;      // optional FP32 shadow, same layout
     int64_t n, p;
+    bool sym;
     __device__ __forceinline__ void put(int k, double v) const {
         if (ds) ds[p * 36 + k] = v;
         if (ds32) ds32[p * 36 + k] = (float)v;
@@ -336,14 +346,21 @@ __global__ void __launch_bounds__(kCBlock) k_constitutive_3d(const double *__res
             if (branch) branch[p0 + t] = (int8_t)b;
         }
         __syncthreads();
-        if (want) {
+        if (want && !w.sym) {
             for (int i = t; i < 36 * np; i += kCBlock) {
                 const double v = sds[(i / 36) * kDStride + i % 36];
                 if (w.ds) w.ds[p0 * 36 + i] = v;
                 if (w.ds32) w.ds32[p0 * 36 + i] = (float)v;
             }
-            __syncthreads();   // tangent rows read out: the area now takes the stresses
+        } else if (want) {
+            for (int i = t; i < 21 * np; i += kCBlock) {
+                const int pt = i / 21;
+                const double v = sds[pt * kDStride + sym_full(i - 21 * pt)];
+                if (w.ds) w.ds[p0 * 21 + i] = v;
+                if (w.ds32) w.ds32[p0 * 21 + i] = (float)v;
+            }
         }
+        if (want) __syncthreads();   // tangent rows read out: the area now takes the stresses
         if (t < np) {
 #pragma unroll
             for (int c = 0; c < 6; ++c) sds[t * 7 + c] = s[c];
@@ -354,9 +371,10 @@ __global__ void __launch_bounds__(kCBlock) k_constitutive_3d(const double *__res
     }
 }
 
-// Persistent grid of the 3D return mapping: 16 workgroups per CU (4 resident; measured best of
-// 4-16 on MI355X: 190 -> 160 us on 1.29M points against one workgroup per point block).
-constexpr int kCBlocksPerCU = 16;
+// Persistent grid of the 3D return mapping: 32 workgroups per CU (4 resident).  Measured on MI355X
+// on 1.29M points: 190 us with one workgroup per point block and no prefetch, 165 / 160 / 153 us
+// at 8 / 16 / 24-64 workgroups per CU.
+constexpr int kCBlocksPerCU = 32;
 
 unsigned const3d_grid(int64_t n) {
     static int n_cu = 0;
@@ -579,6 +597,19 @@ __device__ void davis(int type, double c0, double phi, double psi, double lam, d
     }
 }
 
+// Full point-major DS (n x 36) from the packed symmetric one (n x 21): entry s + 6 m takes packed
+// entry (min(s, m), max(s, m)).
+__global__ void __launch_bounds__(256) k_ds_unpack_sym(int64_t n, const double *__restrict__ DSp,
+                                                       double *__restrict__ DS) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * 36) return;
+    const int64_t p = i / 36;
+    const int k = (int)(i - p * 36);
+    const int r = k % 6, c = k / 6;
+    const int s = r < c ? r : c, m = r < c ? c : r;
+    DS[i] = DSp[p * 21 + s * 6 - s * (s - 1) / 2 + (m - s)];
+}
+
 __global__ void k_davis(const double *__restrict__ c0, const double *__restrict__ phi,
                         const double *__restrict__ psi, int64_t n, double lam, int type, double *__restrict__ cb,
                         double *__restrict__ sp) {
@@ -613,10 +644,33 @@ int ssx_constitutive_3d(const double *E, const double *c_bar, const double *sin_
     SSX_REQUIRE(E && S && c_bar && sin_phi && shear && lame, "ssx_constitutive_3d: null pointer");
     if (n == 0) return SSX_OK;
     MatSource m{c_bar, sin_phi, shear, lame, nullptr, nullptr, 1};
-    DSWriter w{DS, DS32, n, 0};
+    DSWriter w{DS, DS32, n, 0, false};
     hipLaunchKernelGGL(k_constitutive_3d, dim3(const3d_grid(n)), dim3(kCBlock), 0, ssx::as_stream(stream), E, m, n,
                        S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d");
+    return SSX_OK;
+}
+
+int ssx_constitutive_3d_sym(const double *E, const double *c_bar, const double *sin_phi, const double *shear,
+                            const double *lame, int64_t n, double *S, double *DSp, int8_t *branch, void *stream) {
+    SSX_REQUIRE(n >= 0, "ssx_constitutive_3d_sym: n < 0");
+    SSX_REQUIRE(E && S && c_bar && sin_phi && shear && lame, "ssx_constitutive_3d_sym: null pointer");
+    if (n == 0) return SSX_OK;
+    MatSource m{c_bar, sin_phi, shear, lame, nullptr, nullptr, 1};
+    DSWriter w{DSp, nullptr, n, 0, true};
+    hipLaunchKernelGGL(k_constitutive_3d, dim3(const3d_grid(n)), dim3(kCBlock), 0, ssx::as_stream(stream), E, m, n,
+                       S, w, branch);
+    SSX_CHECK_LAUNCH("ssx_constitutive_3d_sym");
+    return SSX_OK;
+}
+
+int ssx_ds_unpack_sym(int64_t n, const double *DSp, double *DS, void *stream) {
+    SSX_REQUIRE(n >= 0, "ssx_ds_unpack_sym: n < 0");
+    if (n == 0) return SSX_OK;
+    SSX_REQUIRE(DSp && DS, "ssx_ds_unpack_sym: null pointer");
+    hipLaunchKernelGGL(k_ds_unpack_sym, dim3(ssx::grid_for(n * 36, 256)), dim3(256), 0, ssx::as_stream(stream), n,
+                       DSp, DS);
+    SSX_CHECK_LAUNCH("ssx_ds_unpack_sym");
     return SSX_OK;
 }
 
@@ -626,7 +680,7 @@ int ssx_constitutive_3d_mat(const double *E, const int32_t *mat_id, const double
     SSX_REQUIRE(E && S && mat_id && table && n_mat > 0, "ssx_constitutive_3d_mat: null pointer");
     if (n == 0) return SSX_OK;
     MatSource m{nullptr, nullptr, nullptr, nullptr, mat_id, table, n_q};
-    DSWriter w{DS, DS32, n, 0};
+    DSWriter w{DS, DS32, n, 0, false};
     hipLaunchKernelGGL(k_constitutive_3d, dim3(const3d_grid(n)), dim3(kCBlock), 0, ssx::as_stream(stream), E, m, n,
                        S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d_mat");

@@ -223,15 +223,18 @@ class DevicePattern:
         return ke
 
     def element_blocks_3d(self, DS: torch.Tensor, r: float = 0.0, mat=None,
-                          branch: torch.Tensor | None = None) -> torch.Tensor:
+                          branch: torch.Tensor | None = None, sym: bool = False) -> torch.Tensor:
         """(n_elem, 55, 3, 3) upper node-pair blocks of the P2 element tangents; `branch` = the return
-        mapping's per-point branch ids for this DS (elastic points then take the closed form)."""
+        mapping's per-point branch ids for this DS (elastic points then take the closed form); `sym`:
+        DS is the packed (n_int, 21) symmetric tangent (constant-Jacobian meshes only)."""
+        if sym and not self.affine:
+            raise ValueError("a packed symmetric DS needs the constant-Jacobian form of the mesh")
         if self._kb is None:
             self._kb = torch.empty((self.n_elem, 55, 3, 3), dtype=torch.float64, device=DS.device)
         sh, la = (None, None) if mat is None else mat
         if self.affine:
-            _lib.call("ssx_element_blocks_affine_3d", self.n_p, ptr(self.gm), ptr(self.gref), ptr(DS), self.n_elem,
-                      self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
+            _lib.call("ssx_element_blocks_affine_3d", self.n_p, ptr(self.gm), ptr(self.gref), ptr(DS), int(sym),
+                      self.n_elem, self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
         else:
             _lib.call("ssx_element_blocks_3d", self.n_p, ptr(self.dphi), ptr(DS), ptr(self.weight), self.n_elem,
                       self.n_q, float(r), ptr(sh), ptr(la), ptr(branch), ptr(self._kb), stream_ptr())
@@ -258,17 +261,21 @@ class DevicePattern:
     tangent_method = "nodes"
 
     def tangent(self, DS: torch.Tensor, out: torch.Tensor, out32: torch.Tensor | None = None, r: float = 0.0,
-                elastic: torch.Tensor | None = None, mat=None, branch: torch.Tensor | None = None) -> None:
+                elastic: torch.Tensor | None = None, mat=None, branch: torch.Tensor | None = None,
+                sym: bool = False) -> None:
         """Tangent values on the pattern.  K_r = (1-r) K + r K_el either from the assembled elastic
         values (`elastic`) or folded into the point tangents (`mat` = (shear, lame) per point).
-        `branch` (P2 tetrahedra, with `mat`): the return mapping's branch ids for this DS."""
+        `branch` (P2 tetrahedra, with `mat`): the return mapping's branch ids for this DS.  `sym`:
+        DS is packed symmetric (n_int, 21) (P2 tetrahedra in the constant-Jacobian form)."""
+        if sym and not (self.affine and self.ns is not None and self.tangent_method == "nodes"):
+            raise ValueError("a packed symmetric DS needs the P2 block path in the constant-Jacobian form")
         if mat is not None and self.ns is None:
             raise ValueError("the K_r fold needs the node-gather path (3D tetrahedra)")
         if branch is not None and mat is None:
             raise ValueError("branch ids need the material (mat = (shear, lame))")
         if self.ns is not None and (self.n_p == 35 or self.tangent_method == "nodes"):
             if self.n_p == 10 and self.n_q == 11:
-                kb = self.element_blocks_3d(DS, r, mat, branch)
+                kb = self.element_blocks_3d(DS, r, mat, branch, sym)
                 self.gather_blocks(kb, out, out32, r, None if mat is not None else elastic)
                 return
             if self.n_p == 35:

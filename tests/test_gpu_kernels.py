@@ -55,6 +55,35 @@ def test_constitutive_3d_stratified_vs_oracle(gpu, n):
     np.testing.assert_array_equal(DS32.cpu().numpy(), DS.cpu().numpy().astype(np.float32))
 
 
+def test_constitutive_3d_packed_symmetric_tangent(gpu):
+    """The packed return mapping writes the full layout's upper entries (s <= m) bit for bit, with
+    the same stresses and branch ids; the unpacked tangent equals the full one (the tangent is
+    symmetric) to 1e-14 of its largest entry per point."""
+    from slope_stability_petsc4py_amd.device import ptr
+    print("case: constitutive_3d packed symmetric DS", flush=True)
+    n = 50_001
+    E, _b = W.stratified_strains(n, seed=4)
+    mat = [np.full(n, 4 * np.sqrt(3)), np.full(n, 1 / np.sqrt(3)), np.full(n, W.HOMO.shear), np.full(n, W.HOMO.lame)]
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to("cuda")  # noqa: E731
+    Ed, md = t(E), [t(a) for a in mat]
+    S1 = torch.empty((n, 6), dtype=torch.float64, device="cuda")
+    S2 = torch.empty_like(S1)
+    DS = torch.empty((n, 36), dtype=torch.float64, device="cuda")
+    DSp = torch.empty((n, 21), dtype=torch.float64, device="cuda")
+    b1 = torch.empty(n, dtype=torch.int8, device="cuda")
+    b2 = torch.empty_like(b1)
+    _lib.call("ssx_constitutive_3d", ptr(Ed), *(ptr(a) for a in md), n, ptr(S1), ptr(DS), None, ptr(b1), 0)
+    _lib.call("ssx_constitutive_3d_sym", ptr(Ed), *(ptr(a) for a in md), n, ptr(S2), ptr(DSp), ptr(b2), 0)
+    full = [s_ + 6 * m_ for s_ in range(6) for m_ in range(s_, 6)]
+    assert torch.equal(S1, S2) and torch.equal(b1, b2)
+    assert torch.equal(DSp, DS[:, full])
+    DSu = torch.full_like(DS, np.nan)
+    _lib.call("ssx_ds_unpack_sym", n, ptr(DSp), ptr(DSu), 0)
+    assert torch.equal(DSu[:, full], DS[:, full])
+    err = _rowrel(DSu.cpu().numpy(), DS.cpu().numpy())
+    assert err.max() < 1e-14, err.max()
+
+
 def test_constitutive_3d_s_only_matches_sds(gpu):
     E, _ = W.stratified_strains(5000, seed=2)
     mat = (np.full(5000, 4 * np.sqrt(3)), np.full(5000, 1 / np.sqrt(3)), np.full(5000, W.HOMO.shear),
@@ -294,6 +323,9 @@ def test_operator_api_3d(gpu):
     assert _rel(F.reshape(-1, order="F"), Fo) < 1e-10
     Ko = oracle.tangent_3d_rows(geom.dphi, DS, geom.weight, pat, geom.n_q)
     assert _rel(K.values.cpu().numpy(), Ko) < 1e-10
+    # P2 slope: straight-sided tetrahedra, the tangent kept packed; DS is unpacked on request
+    assert op.pattern.affine and op._sym
+    assert _rel(op.DS.t().cpu().numpy(), DS) < 1e-10 and op.DS.shape == (36, geom.n_int)
     Ff = op.build_F_free()
     np.testing.assert_array_equal(Ff, F.reshape(-1, order="F")[pat.free_idx])
     Kr = op.build_K_regularized(0.25)
@@ -668,6 +700,14 @@ def test_p2_affine_geometry_kernels_vs_oracle(gpu, kind):
         # without branch ids and without the fold
         dp.tangent(DSd, K2, r=r, mat=(t(G), t(lam)))
         assert _rel(K.cpu().numpy(), K2.cpu().numpy()) < 1e-13
+        # the packed symmetric tangent (21 values per point)
+        DSp = DSd[:, [s_ + 6 * m_ for s_ in range(6) for m_ in range(s_, 6)]].contiguous()
+        dp.tangent(DSp, K2, K32, r=r, mat=(t(G), t(lam)), branch=br, sym=True)
+        assert _rel(K2.cpu().numpy(), exp) < 1e-12
+        assert _rel(K.cpu().numpy(), K2.cpu().numpy()) < 1e-13
+        np.testing.assert_array_equal(K32.cpu().numpy(), K2.cpu().numpy().astype(np.float32))
+        with pytest.raises(ValueError):
+            pts.tangent(DSp, K2, r=r, mat=(t(G), t(lam)), branch=br, sym=True)
     K = torch.empty(pat.nnz, dtype=torch.float64, device=dev)
     dp.tangent(DSd, K)
     exp = Kt.copy()
