@@ -107,7 +107,7 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
     constexpr int OFF = kStG;
     constexpr int NREG = GEO ? 0 : kStRegs;
     constexpr int NQN = kNQ * kNP;                 // 110 values per component
-    __shared__ double sg[2][OFF + 4 * kNQ];
+    __shared__ double sg[2][OFF + 4 * kNQ + 1];   // + 1: the element's material is uniform (EL)
     __shared__ double sh[GEO ? kStG + kNQ : 1];   // GEO: h_k(q, n) at k NQN + q kNP + n, then wf
     const int lane = threadIdx.x;
     int p, a, b;
@@ -173,6 +173,12 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
             sg[bb][OFF + 2 * kNQ + lane] = rl;
             sg[bb][OFF + 3 * kNQ + lane] = rb == 0 ? 1.0 : 0.0;
         }
+        if constexpr (EL) {
+            // lanes past the 11 points hold point 0's values
+            const double s0 = lane_value(rs, 0), l0 = lane_value(rl, 0);
+            const bool uni = __all(rs == s0 && rl == l0);
+            if (lane == 0) sg[bb][OFF + 4 * kNQ] = uni ? 1.0 : 0.0;
+        }
     };
     fetch(e);
     stage(0);
@@ -197,20 +203,50 @@ k_ke_blocks_p2(const double *__restrict__ dphi, const double *__restrict__ DS, c
         // compiler keep the two paths' accumulators in different registers and copy all nine
         // between them at the join and on the back edge, 18 moves per point.)
         if constexpr (EL) {
+            if (L[OFF + 4 * kNQ] != 0.0) {
+                // One material over the element (uniform): M = sum over the elastic points of
+                // w g_a g_b^T (9 FMAs per point), then the closed form once,
+                // lame M + G (M^T + tr(M) I).
+                double M[kBlk];
+#pragma unroll
+                for (int k = 0; k < kBlk; ++k) M[k] = 0.0;
 #pragma unroll 1
-            for (int q = 0; q < kNQ; ++q) {
-                if (L[OFF + 3 * kNQ + q] == 0.0) continue;   // uniform: one point for the whole wavefront
-                const double wq = L[OFF + q];
-                double ha[3], gb[3];
-                grads(q, ha, gb);
-                const double wl = wq * L[OFF + 2 * kNQ + q], wg = wq * L[OFF + kNQ + q];
-                const double dg = wg * fma(ha[2], gb[2], fma(ha[1], gb[1], ha[0] * gb[0]));
+                for (int q = 0; q < kNQ; ++q) {
+                    if (L[OFF + 3 * kNQ + q] == 0.0) continue;   // uniform: one point for the whole wavefront
+                    const double wq = L[OFF + q];
+                    double ha[3], gb[3];
+                    grads(q, ha, gb);
 #pragma unroll
-                for (int i = 0; i < 3; ++i) {
-                    const double li = wl * ha[i], gi = wg * gb[i];
+                    for (int i = 0; i < 3; ++i) {
+                        const double wa = wq * ha[i];
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) acc[3 * i + j] = fma(gi, ha[j], fma(li, gb[j], acc[3 * i + j]));
-                    acc[4 * i] += dg;
+                        for (int j = 0; j < 3; ++j) M[3 * i + j] = fma(wa, gb[j], M[3 * i + j]);
+                    }
+                }
+                const double lm = L[OFF + 2 * kNQ], gs = L[OFF + kNQ];
+                const double tr = M[0] + M[4] + M[8];
+#pragma unroll
+                for (int i = 0; i < 3; ++i)
+#pragma unroll
+                    for (int j = 0; j < 3; ++j)
+                        acc[3 * i + j] = fma(lm, M[3 * i + j], gs * (M[3 * j + i] + (i == j ? tr : 0.0)));
+            } else {
+#pragma unroll 1
+                for (int q = 0; q < kNQ; ++q) {
+                    if (L[OFF + 3 * kNQ + q] == 0.0) continue;   // uniform: one point for the whole wavefront
+                    const double wq = L[OFF + q];
+                    double ha[3], gb[3];
+                    grads(q, ha, gb);
+                    const double wl = wq * L[OFF + 2 * kNQ + q], wg = wq * L[OFF + kNQ + q];
+                    const double dg = wg * fma(ha[2], gb[2], fma(ha[1], gb[1], ha[0] * gb[0]));
+#pragma unroll
+                    for (int i = 0; i < 3; ++i) {
+                        const double li = wl * ha[i], gi = wg * gb[i];
+#pragma unroll
+                        for (int j = 0; j < 3; ++j)
+                            acc[3 * i + j] = fma(gi, ha[j], fma(li, gb[j], acc[3 * i + j]));
+                        acc[4 * i] += dg;
+                    }
                 }
             }
         }

@@ -56,9 +56,12 @@ def test_constitutive_3d_stratified_vs_oracle(gpu, n):
 
 
 def test_constitutive_3d_packed_symmetric_tangent(gpu):
-    """The packed return mapping writes the full layout's upper entries (s <= m) bit for bit, with
-    the same stresses and branch ids; the unpacked tangent equals the full one (the tangent is
-    symmetric) to 1e-14 of its largest entry per point."""
+    """The packed return mapping writes the full layout's upper entries (s <= m), with the same
+    stresses and branch ids bit for bit; its tangent entries agree with the full kernel's to 1e-14
+    of the point's largest entry (the packed kernel forms only the 21 upper entries, so a product
+    that the full kernel shares between (i, j) and (j, i) may fuse into an FMA there: last-bit
+    differences), and the unpacked tangent equals the full one (the tangent is symmetric) to the
+    same bound.  Both are checked against the oracle to 1e-10 elsewhere."""
     from slope_stability_petsc4py_amd.device import ptr
     print("case: constitutive_3d packed symmetric DS", flush=True)
     n = 50_001
@@ -76,11 +79,15 @@ def test_constitutive_3d_packed_symmetric_tangent(gpu):
     _lib.call("ssx_constitutive_3d_sym", ptr(Ed), *(ptr(a) for a in md), n, ptr(S2), ptr(DSp), ptr(b2), 0)
     full = [s_ + 6 * m_ for s_ in range(6) for m_ in range(s_, 6)]
     assert torch.equal(S1, S2) and torch.equal(b1, b2)
-    assert torch.equal(DSp, DS[:, full])
+    DSh = DS.cpu().numpy()
+    errp = _rowrel(DSp.cpu().numpy(), DSh[:, full])
+    print(f"  packed vs full upper entries: max per-point rel {errp.max():.2e}, "
+          f"bitwise rows {(errp == 0).mean():.4f}", flush=True)
+    assert errp.max() < 1e-14, errp.max()
     DSu = torch.full_like(DS, np.nan)
     _lib.call("ssx_ds_unpack_sym", n, ptr(DSp), ptr(DSu), 0)
-    assert torch.equal(DSu[:, full], DS[:, full])
-    err = _rowrel(DSu.cpu().numpy(), DS.cpu().numpy())
+    assert torch.equal(DSu[:, full], DSp)
+    err = _rowrel(DSu.cpu().numpy(), DSh)
     assert err.max() < 1e-14, err.max()
 
 
