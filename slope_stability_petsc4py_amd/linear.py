@@ -17,7 +17,8 @@ reference's free-dof system).
 Preconditioners
   * ``PMultigrid``: V-cycle P2 -> P1 (vertex dofs), Chebyshev(Jacobi) smoothing in FP32
     (values32) or FP64, Galerkin coarse operator P^T A P formed on the device, inverted densely
-    (FP64 Cholesky, explicit inverse; a coarse solve is one matrix-vector product).  Replaces
+    by the library's blocked Cholesky, triangular inverse and W^T W in the preconditioner's
+    precision (a coarse solve is one symmetric matrix-vector product).  Replaces
     Hypre BoomerAMG / the manual p-MG shell (linear/solver.py:96-779).  Above a size limit the P1
     level stays sparse (Galerkin product on the P1 pattern, Chebyshev smoothing) and the dense
     level below it holds rigid-body modes on node aggregates.
