@@ -254,15 +254,21 @@ __global__ void __launch_bounds__(256) k_spd_potrf(int64_t n_pad, int k, T *__re
         Mk[(int64_t)i * n_pad + l] = l <= i ? S[i * kTP + l] : (T)0;
     }
     // in-place inverse of the lower triangle, columns right to left: with the trailing block
-    // already inverted (X22), column j of the inverse is -X22 L[j+1:, j] / L[j][j]
+    // already inverted (X22), column j of the inverse is -X22 L[j+1:, j] / L[j][j].  Row i's sum
+    // over m = j+1 .. i is split over the four lanes 4 i + h (m = j + 1 + h, j + 5 + h, ...) and
+    // combined by two cross-lane exchanges, so no lane walks a chain longer than 16 products (one
+    // lane per row walked up to 63, one dependent LDS load and FMA after another).
+    const int ri = tid >> 2, h = tid & 3;
     for (int j = kT - 1; j >= 0; --j) {
         __syncthreads();
         const T inv = (T)1 / S[j * kTP + j];
         T y = 0;
-        if (tid > j && tid < kT)
-            for (int m = j + 1; m <= tid; ++m) y += S[tid * kTP + m] * S[m * kTP + j];
+        if (ri > j)
+            for (int m = j + 1 + h; m <= ri; m += 4) y += S[ri * kTP + m] * S[m * kTP + j];
+        y += __shfl_xor(y, 1);
+        y += __shfl_xor(y, 2);
         __syncthreads();
-        if (tid > j && tid < kT) S[tid * kTP + j] = -inv * y;
+        if (ri > j && h == 0) S[ri * kTP + j] = -inv * y;
         if (tid == 0) S[j * kTP + j] = inv;
     }
     __syncthreads();

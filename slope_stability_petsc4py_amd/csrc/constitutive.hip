@@ -62,15 +62,11 @@ __device__ __forceinline__ void eig_proj(const double esq[6], const double et[6]
 // Packed symmetric layout of a 6x6 tangent: the 21 entries DS[s + 6 m], s <= m, row by row
 // (s = 0: m = 0..5, s = 1: m = 1..5, ...); row s starts at packed index 6 s - s (s - 1) / 2.
 
+// Tangent outputs of a return-mapping launch: point-major DS (n x 36, or n x 21 packed for the
+// packed kernel) and its optional FP32 copy in the same layout; both may be null (stresses only).
 struct DSWriter {
-    double *ds;       // point-major DS (n x 36; n x 21 packed when sym) or nullptr
-    float *ds32;      // optional FP32 shadow, same layout
-    int64_t n, p;
-    bool sym;
-    __device__ __forceinline__ void put(int k, double v) const {
-        if (ds) ds[p * 36 + k] = v;
-        if (ds32) ds32[p * 36 + k] = (float)v;
-    }
+    double *ds;
+    float *ds32;
     __device__ __forceinline__ bool on() const { return ds != nullptr || ds32 != nullptr; }
 };
 
@@ -662,7 +658,7 @@ int ssx_constitutive_3d(const double *E, const double *c_bar, const double *sin_
     SSX_REQUIRE(E && S && c_bar && sin_phi && shear && lame, "ssx_constitutive_3d: null pointer");
     if (n == 0) return SSX_OK;
     MatSource m{c_bar, sin_phi, shear, lame, nullptr, nullptr, 1};
-    DSWriter w{DS, DS32, n, 0, false};
+    DSWriter w{DS, DS32};
     hipLaunchKernelGGL(k_constitutive_3d<false>, dim3(const3d_grid(n)), dim3(kCBlock), 0, ssx::as_stream(stream), E, m, n,
                        S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d");
@@ -675,7 +671,7 @@ int ssx_constitutive_3d_sym(const double *E, const double *c_bar, const double *
     SSX_REQUIRE(E && S && c_bar && sin_phi && shear && lame, "ssx_constitutive_3d_sym: null pointer");
     if (n == 0) return SSX_OK;
     MatSource m{c_bar, sin_phi, shear, lame, nullptr, nullptr, 1};
-    DSWriter w{DSp, nullptr, n, 0, true};
+    DSWriter w{DSp, nullptr};
     hipLaunchKernelGGL(k_constitutive_3d<true>, dim3(const3d_grid(n)), dim3(kCBlock), 0, ssx::as_stream(stream), E, m, n,
                        S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d_sym");
@@ -698,7 +694,7 @@ int ssx_constitutive_3d_mat(const double *E, const int32_t *mat_id, const double
     SSX_REQUIRE(E && S && mat_id && table && n_mat > 0, "ssx_constitutive_3d_mat: null pointer");
     if (n == 0) return SSX_OK;
     MatSource m{nullptr, nullptr, nullptr, nullptr, mat_id, table, n_q};
-    DSWriter w{DS, DS32, n, 0, false};
+    DSWriter w{DS, DS32};
     hipLaunchKernelGGL(k_constitutive_3d<false>, dim3(const3d_grid(n)), dim3(kCBlock), 0, ssx::as_stream(stream), E, m, n,
                        S, w, branch);
     SSX_CHECK_LAUNCH("ssx_constitutive_3d_mat");
