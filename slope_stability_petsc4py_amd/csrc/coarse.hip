@@ -221,61 +221,79 @@ __global__ void k_spd_load(int64_t n, int64_t n_pad, const double *__restrict__ 
     W[g] = (T)(i == j ? 1 : 0);
 }
 
-// Diagonal block k: L_kk (Cholesky in LDS, written to M) and its inverse (to Dinv[k]).
+// Diagonal block k: L_kk (Cholesky in LDS, written to M) and its inverse (to Dinv[k]).  One
+// barrier per column in each phase:
+//   factorisation: right-looking without the column scaling.  After step j the trailing entries
+//     hold A - sum_{m <= j} a_m a_m^T / d_m (a_m the unscaled column m, d_m its pivot), so a step
+//     reads column j and writes only entries right of it; the columns are scaled by 1 / sqrt(d_m)
+//     at the end;
+//   inverse X = L^-1, columns right to left: X[i][j] = -(1 / L[j][j]) sum_{m=j+1..i} X[i][m] L[m][j].
+//     X is written transposed into the upper triangle (X[i][m] at S[m][i], its diagonal on S's
+//     diagonal; L's diagonal is kept in sq[]), so a step writes row j and reads only rows below it.
+//     Row i's sum is split over the four lanes 4 i + h (m = j + 1 + h, j + 5 + h, ...) and combined
+//     by two cross-lane exchanges.
+// A non-positive pivot is reported as its (1-based) row in info and replaced by 1.
 template <typename T>
 __global__ void __launch_bounds__(256) k_spd_potrf(int64_t n_pad, int k, T *__restrict__ M, T *__restrict__ Dinv,
                                                   int *__restrict__ info) {
     __shared__ T S[kT * kTP];
+    __shared__ T dg[kT], sq[kT];
     const int tid = threadIdx.x;
     T *Mk = M + (int64_t)k * kT * n_pad + (int64_t)k * kT;
     for (int e = tid; e < kT * kT; e += 256) S[(e >> 6) * kTP + (e & 63)] = Mk[(int64_t)(e >> 6) * n_pad + (e & 63)];
     for (int j = 0; j < kT; ++j) {
         __syncthreads();
-        if (tid == 0) {
-            T d = S[j * kTP + j];
-            if (!(d > (T)0)) {
-                if (*info == 0) *info = k * kT + j + 1;
-                d = (T)1;
+        T d = S[j * kTP + j];   // the same for every thread
+        if (!(d > (T)0)) {
+            if (tid == 0 && *info == 0) *info = k * kT + j + 1;
+            d = (T)1;
+        }
+        if (tid == 0) dg[j] = d;
+        // this thread: column l of rows w, w + 4, ..., w + 60; its 33 operands are loaded before any
+        // of its 16 stores (the stores could alias later loads as far as the compiler can tell, so
+        // an update loop would wait for each store before the next loads)
+        const int l = tid & 63, w = tid >> 6;
+        if (l > j) {
+            const T b = S[l * kTP + j] / d;
+            T a[kT / 4], c[kT / 4];
+#pragma unroll
+            for (int r = 0; r < kT / 4; ++r) {
+                a[r] = S[(w + 4 * r) * kTP + j];
+                c[r] = S[(w + 4 * r) * kTP + l];
             }
-            S[j * kTP + j] = sqrt(d);
+#pragma unroll
+            for (int r = 0; r < kT / 4; ++r)
+                if (l <= w + 4 * r) S[(w + 4 * r) * kTP + l] = c[r] - a[r] * b;
         }
-        __syncthreads();
-        const T djj = S[j * kTP + j];
-        if (tid > j && tid < kT) S[tid * kTP + j] /= djj;
-        __syncthreads();
-        for (int e = tid; e < kT * kT; e += 256) {
-            const int i = e >> 6, l = e & 63;
-            if (l > j && l <= i) S[i * kTP + l] -= S[i * kTP + j] * S[l * kTP + j];
-        }
+    }
+    __syncthreads();
+    if (tid < kT) sq[tid] = sqrt(dg[tid]);
+    __syncthreads();
+    for (int e = tid; e < kT * kT; e += 256) {
+        const int i = e >> 6, l = e & 63;
+        if (l < i) S[i * kTP + l] /= sq[l];
     }
     __syncthreads();
     for (int e = tid; e < kT * kT; e += 256) {
         const int i = e >> 6, l = e & 63;
-        Mk[(int64_t)i * n_pad + l] = l <= i ? S[i * kTP + l] : (T)0;
+        Mk[(int64_t)i * n_pad + l] = l < i ? S[i * kTP + l] : (l == i ? sq[i] : (T)0);
     }
-    // in-place inverse of the lower triangle, columns right to left: with the trailing block
-    // already inverted (X22), column j of the inverse is -X22 L[j+1:, j] / L[j][j].  Row i's sum
-    // over m = j+1 .. i is split over the four lanes 4 i + h (m = j + 1 + h, j + 5 + h, ...) and
-    // combined by two cross-lane exchanges, so no lane walks a chain longer than 16 products (one
-    // lane per row walked up to 63, one dependent LDS load and FMA after another).
     const int ri = tid >> 2, h = tid & 3;
     for (int j = kT - 1; j >= 0; --j) {
-        __syncthreads();
-        const T inv = (T)1 / S[j * kTP + j];
+        const T inv = (T)1 / sq[j];
         T y = 0;
         if (ri > j)
-            for (int m = j + 1 + h; m <= ri; m += 4) y += S[ri * kTP + m] * S[m * kTP + j];
+            for (int m = j + 1 + h; m <= ri; m += 4) y += S[m * kTP + ri] * S[m * kTP + j];
         y += __shfl_xor(y, 1);
         y += __shfl_xor(y, 2);
-        __syncthreads();
-        if (ri > j && h == 0) S[ri * kTP + j] = -inv * y;
+        if (ri > j && h == 0) S[j * kTP + ri] = -inv * y;
         if (tid == 0) S[j * kTP + j] = inv;
+        __syncthreads();
     }
-    __syncthreads();
     T *D = Dinv + (int64_t)k * kT * kT;
     for (int e = tid; e < kT * kT; e += 256) {
         const int i = e >> 6, l = e & 63;
-        D[e] = l <= i ? S[i * kTP + l] : (T)0;
+        D[e] = l <= i ? S[l * kTP + i] : (T)0;
     }
 }
 
