@@ -54,8 +54,10 @@ int ssx_constitutive_3d_mat(const double *E, const int32_t *mat_id, const double
                             int64_t n, double *S, double *DS, float *DS32, int8_t *branch, void *stream);
 
 /* ssx_constitutive_3d with the tangent written packed: DSp (n x 21) holds DS[s + 6 m] for s <= m,
- * row by row (s = 0: m = 0..5; s = 1: m = 1..5; ...), the same values as the full layout's upper
- * entries.  The associated-flow tangent is symmetric (sin psi = sin phi in the reference kernel,
+ * row by row (s = 0: m = 0..5; s = 1: m = 1..5; ...): the full layout's upper entries to within
+ * the last bit (only the 21 entries are formed, so a product the full kernel shares between (i, j)
+ * and (j, i) may fuse into an FMA here; at most 2.6e-16 of the point's largest entry), with the
+ * same stresses and branch ids bit for bit.  The associated-flow tangent is symmetric (sin psi = sin phi in the reference kernel,
  * cython/constitutive_3D_kernel.h:73), so the 21 entries carry it whole; no counterpart in the
  * reference, for callers that hand DS straight to ssx_element_blocks_affine_3d (ds_sym = 1). */
 int ssx_constitutive_3d_sym(const double *E, const double *c_bar, const double *sin_phi, const double *shear,
