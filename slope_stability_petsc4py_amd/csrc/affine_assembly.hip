@@ -74,18 +74,31 @@ __global__ void __launch_bounds__(kBlock) k_strain_affine(const double *__restri
 }
 
 // Force, stage 1: element-local vector f_e[a] = sum_q w_q B_a(q)^T S_q, one lane per (element, node)
-// (the same sums, in the same order, as the per-point kernel); stage 2 is the node gather.
+// (the same sums, in the same order, as the per-point kernel); stage 2 is the node gather.  The
+// stresses and factors of the block's elements (at most kEpb) are staged in LDS by coalesced loads
+// first: read directly, every lane issued 76 loads of values its element's other lanes read too.
+constexpr int kEpb = kBlock / kNP + 2;   // elements a block's lanes can touch
+
 __global__ void __launch_bounds__(kBlock) k_force_affine(const double *__restrict__ gm, const double *__restrict__ ref,
                                                          const double *__restrict__ S, int64_t n_elem,
                                                          double *__restrict__ fe) {
     __shared__ double sh[kTab + kNQ];
+    __shared__ double ss[kEpb * kNQ * kNS];
+    __shared__ double sgm[kEpb * kGW];
     for (int i = threadIdx.x; i < kTab + kNQ; i += kBlock) sh[i] = ref[i];
+    const int64_t t0 = (int64_t)blockIdx.x * kBlock;
+    const int64_t t_end = t0 + kBlock < n_elem * kNP ? t0 + kBlock : n_elem * kNP;
+    const int64_t e0 = t0 / kNP;
+    const int ne = (int)((t_end - 1) / kNP + 1 - e0);
+    for (int i = threadIdx.x; i < ne * kNQ * kNS; i += kBlock) ss[i] = S[e0 * (kNQ * kNS) + i];
+    for (int i = threadIdx.x; i < ne * kGW; i += kBlock) sgm[i] = gm[e0 * kGW + i];
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    const int64_t t = t0 + threadIdx.x;
     if (t >= n_elem * kNP) return;
     const int64_t e = t / kNP;
     const int node = (int)(t - e * kNP);
-    const double *G = gm + e * kGW;
+    const int le = (int)(e - e0);
+    const double *G = sgm + le * kGW;
     double m[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) m[k] = G[k];
@@ -98,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) k_force_affine(const double *__restric
         const double gy = fma(m[5], r2, fma(m[4], r1, m[3] * r0));
         const double gz = fma(m[8], r2, fma(m[7], r1, m[6] * r0));
         const double wg = det * sh[kTab + q];
-        const double *s = S + (e * kNQ + q) * kNS;
+        const double *s = ss + (le * kNQ + q) * kNS;
         const double s0 = s[0], s1 = s[1], s2 = s[2], s3 = s[3], s4 = s[4], s5 = s[5];
         f0 += wg * (gx * s0 + gy * s3 + gz * s5);
         f1 += wg * (gx * s3 + gy * s1 + gz * s4);
