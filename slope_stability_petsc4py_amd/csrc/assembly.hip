@@ -202,8 +202,22 @@ __global__ void __launch_bounds__(256) k_force_gather_nodes(const double *__rest
     const int64_t node = r / dim;
     const int c = (int)(r - node * dim);
     double acc = 0.0;
+    int32_t s = ns_ptr[node];
     const int32_t s1 = ns_ptr[node + 1];
-    for (int32_t s = ns_ptr[node]; s < s1; ++s) acc += fe[(int64_t)ns_elem[s] * nld + dim * (int)ns_lnode[s] + c];
+    // four slots' loads in flight at a time (one slot's metadata, then its entry, one slot after
+    // another, kept a single chain of dependent loads in flight); the adds stay in slot order, so
+    // the sums are the same bit for bit
+    for (; s + 4 <= s1; s += 4) {
+        int64_t o[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) o[k] = (int64_t)ns_elem[s + k] * nld + dim * (int)ns_lnode[s + k] + c;
+        double v[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) v[k] = fe[o[k]];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) acc += v[k];
+    }
+    for (; s < s1; ++s) acc += fe[(int64_t)ns_elem[s] * nld + dim * (int)ns_lnode[s] + c];
     F[r] = row_free[r] ? acc : 0.0;
 }
 
