@@ -26,34 +26,46 @@ constexpr int kGW = 10;                   // words of G_e
 constexpr int kBlock = 256;
 
 // Strain: one lane per point g = e kNQ + q; the block's strains leave through LDS in one coalesced
-// pass over E[g0 kNS .. (g0 + np) kNS).
+// pass over E[g0 kNS .. (g0 + np) kNS).  The factors of the block's elements (at most kEpbS) and
+// their 30 displacement values are staged in LDS first, each gathered once per element (read
+// directly, every lane issued 69 loads, 30 of them dependent on a dof index, of values its
+// element's other 10 lanes read too).
+constexpr int kEpbS = kBlock / kNQ + 2;   // elements a block's points can touch
+
 __global__ void __launch_bounds__(kBlock) k_strain_affine(const double *__restrict__ gm, const double *__restrict__ ref,
                                                           const int32_t *__restrict__ edofs,
                                                           const double *__restrict__ u, int64_t n_int,
                                                           double *__restrict__ E) {
     __shared__ double sh[kTab];
     __shared__ double so[kBlock * (kNS + 1)];
+    __shared__ double sgm[kEpbS * kGW];
+    __shared__ double su[kEpbS * 3 * kNP];
     const int t = threadIdx.x;
     for (int i = t; i < kTab; i += kBlock) sh[i] = ref[i];
     const int64_t g0 = (int64_t)blockIdx.x * kBlock;
     const int np = (int)(n_int - g0 < kBlock ? n_int - g0 : kBlock);
+    const int64_t e0 = g0 / kNQ;
+    const int ne = (int)((g0 + np - 1) / kNQ + 1 - e0);
+    for (int i = t; i < ne * kGW; i += kBlock) sgm[i] = gm[e0 * kGW + i];
+    for (int i = t; i < ne * 3 * kNP; i += kBlock) su[i] = u[edofs[e0 * (3 * kNP) + i]];
     __syncthreads();
     if (t < np) {
         const int64_t g = g0 + t;
         const int64_t e = g / kNQ;
         const int q = (int)(g - e * kNQ);
-        const double *G = gm + e * kGW;
+        const int le = (int)(e - e0);
+        const double *G = sgm + le * kGW;
         double m[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) m[k] = G[k];
-        const int32_t *dof = edofs + e * (3 * kNP);
+        const double *ue = su + le * (3 * kNP);
         const double *h0 = sh + q * kNP, *h1 = sh + (kNQ + q) * kNP, *h2 = sh + (2 * kNQ + q) * kNP;
         double a[kNS];
 #pragma unroll
         for (int k = 0; k < kNS; ++k) a[k] = 0.0;
 #pragma unroll
         for (int j = 0; j < kNP; ++j) {
-            const double ux = u[dof[3 * j]], uy = u[dof[3 * j + 1]], uz = u[dof[3 * j + 2]];
+            const double ux = ue[3 * j], uy = ue[3 * j + 1], uz = ue[3 * j + 2];
             const double r0 = h0[j], r1 = h1[j], r2 = h2[j];
             const double x = fma(m[2], r2, fma(m[1], r1, m[0] * r0));
             const double y = fma(m[5], r2, fma(m[4], r1, m[3] * r0));
